@@ -1,0 +1,254 @@
+/*
+ * mfic -- MI355X-native monofonIC initial-condition hot path, C ABI.
+ *
+ * Every entry point returns 0 on success or a negative MFIC_ERR_* code;
+ * mfic_last_error() returns a thread-local message for the last failure.
+ * No C++ exception crosses this boundary.  All work is enqueued on the
+ * context's HIP stream in call order; only download / sync / explicit
+ * host-result calls block.  Handles are not thread-safe (as Grid_FFT).
+ *
+ * Reference citations are relative to the monofonIC source tree.
+ *
+ * Data layout (grid_fft.cc:40-112): a grid of logical size n0 x n1 x n2 is
+ * one buffer of n0*n1*(n2+2) reals; in real space element (i,j,k) is at
+ * (i*n1 + j)*(n2+2) + k, in k-space it is the complex array
+ * [n0][n1][n2/2+1] (kx = i, ky = j, kz = k) over the same memory.
+ * FFTs are unitary: both directions scale by 1/sqrt(n0 n1 n2).
+ */
+#ifndef MFIC_H
+#define MFIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- constants --- */
+enum mfic_status {
+    MFIC_OK = 0,
+    MFIC_ERR_ARG = -1,         /* invalid argument (null, mismatched shapes)    */
+    MFIC_ERR_SIZE = -2,        /* unsupported grid size / odd padded size       */
+    MFIC_ERR_HIP = -3,         /* HIP runtime failure                           */
+    MFIC_ERR_ALLOC = -4,       /* device allocation failed                      */
+    MFIC_ERR_STATE = -5,       /* wrong space / unallocated grid                */
+    MFIC_ERR_CONFIG = -6,      /* invalid configuration value                  */
+    MFIC_ERR_UNSUPPORTED = -7  /* feature outside the hot path                  */
+};
+enum mfic_prec { MFIC_F32 = 0, MFIC_F64 = 1 };
+/* same numbering as the reference's space_t (grid_fft.hh:29) */
+enum mfic_space { MFIC_KSPACE = 0, MFIC_RSPACE = 1 };
+/* output operators of operators.hh:25-35 */
+enum mfic_outop { MFIC_OP_ASSIGN = 0, MFIC_OP_ADD = 1, MFIC_OP_SUB = 2, MFIC_OP_MADD = 3 };
+/* operand kinds of the BaseConvolver builders (convolution.hh:74-275) */
+enum mfic_opkind {
+    MFIC_OPND_HESS = 0,      /* (i k_a)(i k_b) f            convolve_Hessians        */
+    MFIC_OPND_SUM_HESS = 1,  /* ((i k_a)(i k_b)+(i k_c)(i k_d)) f  SumOfHessians    */
+    MFIC_OPND_DIFF_HESS = 2, /* ((i k_a)(i k_b)-(i k_c)(i k_d)) f  DifferenceOf...  */
+    MFIC_OPND_GRAD = 3,      /* (i k_a) f                   convolve_Gradients       */
+    MFIC_OPND_IDENT = 4      /* f                                                    */
+};
+enum mfic_lattice { MFIC_LATTICE_SC = 0, MFIC_LATTICE_BCC = 1, MFIC_LATTICE_FCC = 2 };
+
+typedef struct mfic_ctx_s *mfic_ctx;
+typedef struct mfic_grid_s *mfic_grid;
+
+/* ------------------------------------------------------------- context --- */
+const char *mfic_last_error(void);
+const char *mfic_version(void);
+/* context = one HIP device + one stream + twiddle/plan caches.
+ * Replaces the process-global FFTW state set up in main.cc:156-182. */
+int mfic_ctx_create(int device, mfic_ctx *out);
+int mfic_ctx_destroy(mfic_ctx ctx);
+int mfic_ctx_sync(mfic_ctx ctx);
+/* raw hipStream_t of the context (for event timing / interop) */
+int mfic_ctx_stream(mfic_ctx ctx, void **stream);
+
+/* ---------------------------------------------------------------- grid --- */
+/* Grid_FFT<real_t>(N, L, allocate=true, space) + allocate()
+ * (grid_fft.hh:77-83, grid_fft.cc:40-112).  Storage is device memory. */
+int mfic_grid_create(mfic_ctx ctx, const size_t n[3], const double L[3], int prec, int space,
+                     mfic_grid *out);
+/* Grid_FFT::reset / destructor (grid_fft.hh:95-101) */
+int mfic_grid_destroy(mfic_grid g);
+/* Grid_FFT::FourierTransformForward/Backward(false): flip the space flag only
+ * (grid_fft.cc:207-270 with do_transform = false) */
+int mfic_grid_set_space(mfic_grid g, int space);
+int mfic_grid_get_space(mfic_grid g, int *space);
+/* device pointer and element count (n0*n1*(n2+2) reals) */
+int mfic_grid_device_ptr(mfic_grid g, void **ptr, size_t *nreal);
+/* copy the whole buffer in the layout above; host_prec may differ from the
+ * grid precision (converted on the device).  Synchronous. */
+int mfic_grid_upload(mfic_grid g, const void *host, int host_prec);
+int mfic_grid_download(mfic_grid g, void *host, int host_prec);
+/* device-to-device copy, Grid_FFT::copy_from (grid_fft.hh:149-170) */
+int mfic_grid_copy(mfic_grid dst, mfic_grid src);
+/* Grid_FFT::zero (grid_fft.hh:140-147) */
+int mfic_grid_zero(mfic_grid g);
+/* operator*= (grid_fft.hh:400-412): every stored real multiplied by c */
+int mfic_grid_scale(mfic_grid g, double c);
+/* zero_DC_mode, k-space branch (grid_fft.hh:844-850) */
+int mfic_grid_zero_dc(mfic_grid g);
+/* apply_InverseLaplacian (grid_fft.hh:440-449): -f/|k|^2, then DC := 0 */
+int mfic_grid_inverse_laplacian(mfic_grid g);
+/* shift_field (grid_fft.hh:832-842): forward, * exp(i sum_a s_a k_a dx_a),
+ * backward (when transform_back) */
+int mfic_grid_shift(mfic_grid g, const double s[3], int transform_back);
+
+/* ----------------------------------------------------------------- FFT --- */
+/* FourierTransformForward / FourierTransformBackward (grid_fft.cc:207-270):
+ * r2c / c2r with unitary scaling; a no-op when the grid is already in the
+ * target space.  Supported sizes: every n = 2^a 3^b with 4 <= n <= 768 per
+ * axis and n2 even (one-GPU hot-path sizes); others return MFIC_ERR_SIZE. */
+int mfic_fft_forward(mfic_grid g);
+int mfic_fft_backward(mfic_grid g);
+int mfic_fft_size_supported(size_t n);
+
+/* --------------------------------------------------------------- noise --- */
+/* RNG_ngenic constructor seed table (random_ngenic.cc:39-75), host only:
+ * table has n*n entries. */
+int mfic_ngenic_seedtable(long seed, size_t n, uint32_t *table);
+/* RNG_ngenic::Fill_Grid (random_ngenic.cc:84-154), non-distributed branch,
+ * on the device; leaves g in k-space.  RANLUX stream bit-exact with GSL's
+ * ranlxd1; Gaussian transform with correctly rounded log/sin/cos. */
+int mfic_ngenic_fill(mfic_grid g, long seed);
+
+/* ----------------------------------------------------------- cosmology --- */
+/* cosmology::parameters (cosmology_parameters.hh:80-241) inputs; fill with
+ * mfic_cosmo_defaults() (ParameterSet = none, Planck2018EE+BAO+SN defaults,
+ * cosmology_parameters.cc:110-133) and override.  A_s <= -1 selects sigma_8
+ * normalisation, as a given sigma_8 key does (cosmology_parameters.hh:105). */
+typedef struct {
+    double Omega_m, Omega_b, H0, n_s, sigma_8, A_s, k_p;
+    double Tcmb, N_ur, m_nu[3], deg_nu[3], O_nu_norm;
+    double w_0, w_a, Omega_k, alpha_s, beta_s;
+    int zero_radiation;
+    double zstart, ztarget;
+} mfic_cosmo_params;
+/* cosmology::calculator scalars (cosmology_calculator.hh:167-223, 334-432) */
+typedef struct {
+    double h, Omega_nu_massive, Omega_gamma, Omega_r, Omega_c, Omega_DE;
+    double Dnow, Dplus_start, Dplus_target, f_start, vfac_start;
+    double sigma8_measured, pnorm, sqrtpnorm;
+} mfic_cosmo_scalars;
+void mfic_cosmo_defaults(mfic_cosmo_params *p);
+int mfic_cosmo_compute(const mfic_cosmo_params *p, mfic_cosmo_scalars *out);
+/* host evaluations for checks: D+(a), f(a), H(a), E&H T(k) [k in h/Mpc] */
+int mfic_cosmo_eval(const mfic_cosmo_params *p, const char *what, const double *x, double *y,
+                    size_t count);
+
+/* ----------------------------------------------------------------- phi1 --- */
+/* noise normalisation (ic_generator.cc:378-383) + phi1 (ic_generator.cc:
+ * 509-516): phi(k) = -wn(k) A(|k|) / |k|^2 / volfac, DC := 0, where
+ * A = k^{n_s/2} T_EH(k) sqrt(pnorm) (cosmology_calculator.hh:429-432) is
+ * evaluated in FP64 on the device.  wn and phi must be k-space grids of the
+ * same size (phi may equal wn). */
+int mfic_phi1_from_noise(mfic_grid phi, mfic_grid wn, const mfic_cosmo_params *p, int fixing,
+                         int inversion);
+
+/* ------------------------------------------------------------ convolver --- */
+typedef struct {
+    mfic_grid f;     /* k-space N-grid field                      */
+    int kind;        /* mfic_opkind                               */
+    int d[4];        /* directions (HESS: d0 d1; SUM/DIFF: d0..d3) */
+} mfic_operand;
+/* OrszagConvolver<real_t>(N, L) (convolution.hh:446-478): owns the two
+ * padded 3N/2 grids and the N-grid buffer. */
+typedef struct mfic_conv_s *mfic_conv;
+int mfic_conv_create(mfic_ctx ctx, const size_t n[3], const double L[3], int prec, mfic_conv *out);
+int mfic_conv_destroy(mfic_conv c);
+/* OrszagConvolver::convolve2 (convolution.hh:495-518) with the operand
+ * builders of convolution.hh:74-275 and op(out, v) of operators.hh:25-35:
+ * out = / += / -= / += coef * trunc(a * b).  The output grid is left in
+ * k-space. */
+int mfic_conv_convolve2(mfic_conv c, const mfic_operand *a, const mfic_operand *b, mfic_grid out,
+                        int op, double coef);
+/* OrszagConvolver::convolve3 (convolution.hh:525-532): trunc(trunc(a b) c) */
+int mfic_conv_convolve3(mfic_conv c, const mfic_operand *a, const mfic_operand *b,
+                        const mfic_operand *cc, mfic_grid out, int op, double coef);
+
+/* ------------------------------------------------------- LPT potentials --- */
+/* The LPT source block of ic_generator::run (ic_generator.cc:523-629):
+ * phi2 (2LPT), phi3 and A3x/y/z (3LPT) from phi1, each followed by the
+ * inverse Laplacian.  Computed with the reorganised schedule: each Hessian is
+ * transformed to padded real space once, products sharing a truncation are
+ * summed before one r2c, the cubic terms keep their two-stage truncation.
+ * Growth factors are NOT applied here (see mfic_lpt_scale).  Grids not needed
+ * for the order may be NULL. */
+int mfic_lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_grid phi3,
+                     mfic_grid A3x, mfic_grid A3y, mfic_grid A3z);
+
+/* LPT time coefficients (ic_generator.cc:258-273) */
+typedef struct {
+    double Dplus0, vfac, g1, g2, g3, g3c, vfac1, vfac2, vfac3, volfac;
+} mfic_lpt_coef;
+int mfic_lpt_coefficients(const mfic_cosmo_params *p, size_t n, double L, int order,
+                          mfic_lpt_coef *out);
+
+/* ------------------------------------------------------------ PLT tables --- */
+/* particle::lattice_gradient (particle_plt.hh:520-549): tables on an
+ * ng^3 mesh (64 in the reference) for the given lattice. */
+typedef struct mfic_plt_s *mfic_plt;
+int mfic_plt_create(mfic_ctx ctx, int lattice, int ng, mfic_plt *out);
+int mfic_plt_destroy(mfic_plt p);
+/* copy the 4 tables (D_r, D_theta, D_phi, vfac; ng*ng*(ng/2+1) doubles each) */
+int mfic_plt_download(mfic_plt p, double *Dr, double *Dtheta, double *Dphi, double *vfac);
+
+/* -------------------------------------------------------------- assembly --- */
+/* One displacement (comp 0..2) or velocity (comp 3..5) component in k-space
+ * (ic_generator.cc:931-970, 987-1047): gradient of the growth-scaled
+ * potentials plus the A3 curl, PLT velocity correction, unit / L, DC := 0.
+ * Growth factors g1..g3c and vfac1..3 are applied here in FP64 (the
+ * reference applies them to the potentials at ic_generator.cc:644-658,
+ * which is the same product).  plt may be NULL (Fourier gradient,
+ * operators.hh:38-61).  out is left in k-space. */
+int mfic_lpt_component(mfic_ctx ctx, int order, int comp, const mfic_lpt_coef *k, mfic_plt plt,
+                       double unit, mfic_grid phi1, mfic_grid phi2, mfic_grid phi3, mfic_grid A3x,
+                       mfic_grid A3y, mfic_grid A3z, mfic_grid out);
+
+/* ----------------------------------------------------------------- readout --- */
+/* lattice_generator::set_positions / set_velocities for Bravais sc / bcc,
+ * lattice index 0 (particle_generator.hh:499-544, 597-639): `field` is the
+ * real-space component; positions are ((ijk + shift)/N) * lunit + field in
+ * FP64.  dst is a device or host pointer to 3*npart doubles (interleaved
+ * [ipart][3]); is_pos = 0 writes velocities (lunit ignored).  For bcc the
+ * second sub-lattice is read after a Fourier shift by (1/2,1/2,1/2) cells
+ * (the field is left shifted, as the reference leaves it). */
+int mfic_lattice_readout(mfic_grid field, int lattice, int idim, int is_pos, double lunit,
+                         double *dst_device);
+
+/* --------------------------------------------------------------- pipeline --- */
+typedef struct {
+    size_t n;               /* GridRes                                       */
+    double L;               /* BoxLength [Mpc/h]                             */
+    int lpt_order;          /* 1..3 (LPTorder)                               */
+    int lattice;            /* mfic_lattice (ParticleLoad sc / bcc)          */
+    int plt;                /* PLT corrections (ENABLE_PLT)                  */
+    int prec;               /* bulk precision MFIC_F32 / MFIC_F64            */
+    int do_fixing, do_inversion;
+    long seed;              /* [random] seed                                 */
+    int particles;          /* 0: 6 component fields, 1: FP64 particles     */
+    double lunit, vunit;    /* output plugin position_unit / velocity_unit   */
+} mfic_ic_config;
+
+/* The whole hot path of ic_generator::run for one box (ic_generator.cc:
+ * 246-1060): white noise, phi1, LPT sources, assembly, c2r, readout, all on
+ * the device.  Results stay on the device in the run object. */
+typedef struct mfic_run_s *mfic_run;
+int mfic_run_create(mfic_ctx ctx, const mfic_ic_config *cfg, const mfic_cosmo_params *cosmo,
+                    mfic_run *out);
+int mfic_run_destroy(mfic_run r);
+/* execute (enqueue) the whole pipeline; blocking = 0 returns immediately */
+int mfic_run_execute(mfic_run r, int blocking);
+/* component fields (6 grids, dx dy dz vx vy vz, real space) when particles=0 */
+int mfic_run_field(mfic_run r, int comp, mfic_grid *out);
+/* particle arrays (device pointers, 3*npart doubles each) when particles=1 */
+int mfic_run_particles(mfic_run r, double **pos, double **vel, size_t *npart);
+/* potentials after growth scaling (phi1, phi2, phi3, A3x, A3y, A3z) */
+int mfic_run_potential(mfic_run r, int which, mfic_grid *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFIC_H */

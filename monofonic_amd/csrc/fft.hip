@@ -1,0 +1,328 @@
+// 3D real<->complex FFTs on the Grid_FFT layout [n0][n1][n2+2] (gfx950).
+//
+// Forward (r2c, FFTW sign e^{-ikx}): rows pass along the contiguous axis
+// (real rows of n2 -> n2/2+1 complex via a length-n2/2 complex FFT and an
+// untangle step), then two strided pencil passes along axes 1 and 0.
+// Backward (c2r): pencils along axes 0 and 1, then the rows pass.  The
+// unitary scale 1/sqrt(n0 n1 n2) (grid_fft.cc:69, 199-205) is folded into the
+// last pass.  Each pass reads and writes the array once.
+#include <cmath>
+#include <complex>
+
+#include "fft_core.hpp"
+#include "mfic_internal.hpp"
+
+namespace mfic {
+
+// ------------------------------------------------------------ kernels -----
+
+// Transforms of length L along a strided axis.  Block = TK neighbouring
+// pencils (contiguous t) x P threads per pencil; a wavefront reads TK
+// consecutive complex values per pencil position.
+template <typename T, int L, int TK, bool FWD>
+__global__ void __launch_bounds__(Plan<L>::P * TK)
+k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
+          const cx<T>* __restrict__ tw) {
+    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int t = threadIdx.x % TK, q = threadIdx.x / TK;
+    const int tg = blockIdx.x * TK + t;
+    const bool active = tg < tlen;
+    cx<T>* base = data + (long)blockIdx.y * os + tg;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = active ? base[(long)(q + P * e) * ps] : cx<T>{0, 0};
+    fft_regs<T, L, FWD>(v, q, lds, LdsMap{t, TK}, tw);
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) base[(long)(q + P * e) * ps] = cscale(v[e], scale);
+    }
+}
+
+// Real rows of length 2M -> M+1 complex, in place (row pitch in reals).
+template <typename T, int M, int RW>
+__global__ void __launch_bounds__(Plan<M>::P * RW)
+k_rows_r2c(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* __restrict__ tw) {
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int q = threadIdx.x % P, rw = threadIdx.x / P;
+    const long row = (long)blockIdx.x * RW + rw;
+    const bool active = row < nrows;
+    cx<T>* r = reinterpret_cast<cx<T>*>(data + (active ? row : 0) * pitch);
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = active ? r[q + P * e] : cx<T>{0, 0};
+    fft_regs<T, M, true>(v, q, lds, LdsMap{rw * RP, 1}, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[rw * RP + q + P * e] = v[e];
+    __syncthreads();
+    const cx<T>* uw = tw + plan_tw_size<M>();
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            const cx<T> zk = v[e], zm = lds[rw * RP + (k == 0 ? 0 : M - k)];
+            const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+            const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+            const cx<T> X = cadd(Ek, cmul(uw[k], Ok));
+            r[k] = cscale(X, scale);
+        }
+        if (q == 0) {
+            const cx<T> z0 = lds[rw * RP];
+            r[M] = cx<T>{(z0.x - z0.y) * scale, (T)0};
+        }
+    }
+}
+
+// M+1 complex -> real rows of length 2M, in place.  The imaginary parts of
+// the kz = 0 and kz = M terms are discarded (Hermitian projection).
+template <typename T, int M, int RW>
+__global__ void __launch_bounds__(Plan<M>::P * RW)
+k_rows_c2r(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* __restrict__ tw) {
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int q = threadIdx.x % P, rw = threadIdx.x / P;
+    const long row = (long)blockIdx.x * RW + rw;
+    const bool active = row < nrows;
+    cx<T>* r = reinterpret_cast<cx<T>*>(data + (active ? row : 0) * pitch);
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        cx<T> x = active ? r[q + P * e] : cx<T>{0, 0};
+        if (q + P * e == 0) x.y = 0;
+        lds[rw * RP + q + P * e] = x;
+        v[e] = x;
+    }
+    if (q == 0) lds[rw * RP + M] = active ? cx<T>{r[M].x, (T)0} : cx<T>{0, 0};
+    __syncthreads();
+    const cx<T>* uw = tw + plan_tw_size<M>();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = q + P * e;
+        const cx<T> xk = v[e], xm = lds[rw * RP + M - k];
+        // A = X_k + conj(X_{M-k}),  B = (X_k - conj(X_{M-k})) conj(W^k)
+        const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
+        const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
+        const cx<T> B = cmulc(D, uw[k]);
+        v[e] = cx<T>{A.x - B.y, A.y + B.x};  // A + i B
+    }
+    __syncthreads();
+    fft_regs<T, M, false>(v, q, lds, LdsMap{rw * RP, 1}, tw);
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) r[q + P * e] = cscale(v[e], scale);
+        if (q == 0) r[M] = cx<T>{0, 0};
+    }
+}
+
+// ------------------------------------------------------------ tables ------
+
+template <int L> static void fill_tw(std::vector<std::complex<long double>>& out) {
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (int s = 0; s < Plan<L>::S; ++s) {
+        const int R = Plan<L>::r(s), NS = plan_ns<L>(s);
+        for (int k = 0; k < NS; ++k)
+            for (int r = 1; r < R; ++r) {
+                const long double a = -2.0L * PI * (long double)((long)r * k % ((long)NS * R)) / (long double)(NS * R);
+                out.emplace_back(cosl(a), sinl(a));
+            }
+    }
+}
+
+#define MFIC_FOR_SIZES(X) \
+    X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768)
+
+bool fft_length_supported(size_t n) {
+    switch (n) {
+#define X(L) case L:
+        MFIC_FOR_SIZES(X)
+#undef X
+        return true;
+    default:
+        return false;
+    }
+}
+
+static std::vector<std::complex<long double>> make_table(int L, int kind) {
+    std::vector<std::complex<long double>> tw;
+    switch (L) {
+#define X(S) case S: fill_tw<S>(tw); break;
+        MFIC_FOR_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported FFT length " + std::to_string(L));
+    }
+    if (kind == 1) {  // untangle factors exp(-2 pi i k / 2L), k = 0..L
+        const long double PI = 3.141592653589793238462643383279502884L;
+        for (int k = 0; k <= L; ++k) {
+            if (k == 0) { tw.emplace_back(1.0L, 0.0L); continue; }
+            if (k == L) { tw.emplace_back(-1.0L, 0.0L); continue; }
+            if (2 * k == L) { tw.emplace_back(0.0L, -1.0L); continue; }
+            const long double a = -PI * (long double)k / (long double)L;
+            tw.emplace_back(cosl(a), sinl(a));
+        }
+    }
+    return tw;
+}
+
+const void* twiddles(mfic_ctx ctx, int L, int kind, int prec) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto key = std::make_tuple(L, kind, prec);
+    auto it = ctx->tables.find(key);
+    if (it != ctx->tables.end()) return it->second.dev;
+    auto tw = make_table(L, kind);
+    TwTable t;
+    t.count = tw.size();
+    size_t bytes = std::max<size_t>(1, t.count) * (prec == MFIC_F64 ? 16 : 8);
+    MFIC_HIP(hipMalloc(&t.dev, bytes));
+    if (prec == MFIC_F64) {
+        std::vector<cx<double>> h(t.count);
+        for (size_t i = 0; i < t.count; ++i) h[i] = {(double)tw[i].real(), (double)tw[i].imag()};
+        MFIC_HIP(hipMemcpy(t.dev, h.data(), t.count * sizeof(cx<double>), hipMemcpyHostToDevice));
+    } else {
+        std::vector<cx<float>> h(t.count);
+        for (size_t i = 0; i < t.count; ++i) h[i] = {(float)tw[i].real(), (float)tw[i].imag()};
+        MFIC_HIP(hipMemcpy(t.dev, h.data(), t.count * sizeof(cx<float>), hipMemcpyHostToDevice));
+    }
+    ctx->tables[key] = t;
+    return t.dev;
+}
+
+// ------------------------------------------------------------ launchers ---
+
+static int pow2_floor(int x) {
+    int p = 1;
+    while (p * 2 <= x) p *= 2;
+    return p;
+}
+static constexpr size_t kMaxLds = 65536;
+
+template <typename T, int L, bool FWD>
+static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tlen, long nouter, T scale) {
+    constexpr int P = Plan<L>::P;
+    int tk = std::max(4, std::min(64, pow2_floor(256 / P)));
+    while (tk > 1 && (size_t)L * tk * sizeof(cx<T>) > kMaxLds) tk /= 2;
+    while (tk > 1 && P * tk > 1024) tk /= 2;
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    MFIC_CHECK(nouter <= 65535, MFIC_ERR_SIZE, "too many pencil groups");
+    dim3 grid((tlen + tk - 1) / tk, (unsigned)nouter);
+    size_t lds = (size_t)L * tk * sizeof(cx<T>);
+    switch (tk) {
+#define TKCASE(K)                                                                                            \
+    case K:                                                                                                  \
+        if constexpr (P * K <= 1024)                                                                         \
+            hipLaunchKernelGGL((k_pencils<T, L, K, FWD>), grid, dim3(P * K), lds, ctx->stream, data, ps, os, \
+                               tlen, scale, tw);                                                             \
+        break;
+        TKCASE(1) TKCASE(2) TKCASE(4) TKCASE(8) TKCASE(16) TKCASE(32) TKCASE(64)
+#undef TKCASE
+    default:
+        throw Error(MFIC_ERR_SIZE, "bad tile");
+    }
+    MFIC_LAUNCH_CHECK();
+}
+
+template <typename T>
+void fft_pencils(mfic_ctx ctx, void* data, int L, long ps, long os, int tlen, long nouter, bool fwd,
+                 double scale) {
+    cx<T>* d = static_cast<cx<T>*>(data);
+    switch (L) {
+#define X(S)                                                                         \
+    case S:                                                                          \
+        if (fwd) launch_pencils_L<T, S, true>(ctx, d, ps, os, tlen, nouter, (T)scale); \
+        else launch_pencils_L<T, S, false>(ctx, d, ps, os, tlen, nouter, (T)scale);    \
+        break;
+        MFIC_FOR_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported FFT length " + std::to_string(L));
+    }
+}
+
+template <typename T, int M, bool R2C>
+static void launch_rows_M(mfic_ctx ctx, T* data, size_t nrows, long pitch, T scale) {
+    constexpr int P = Plan<M>::P;
+    int rw = std::max(1, std::min(64, pow2_floor(256 / P)));
+    while (rw > 1 && (size_t)(M + 1) * rw * sizeof(cx<T>) > kMaxLds) rw /= 2;
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    dim3 grid((unsigned)((nrows + rw - 1) / rw));
+    size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
+    switch (rw) {
+#define RWCASE(K)                                                                                             \
+    case K:                                                                                                   \
+        if constexpr (P * K <= 1024) {                                                                        \
+            if (R2C)                                                                                          \
+                hipLaunchKernelGGL((k_rows_r2c<T, M, K>), grid, dim3(P * K), lds, ctx->stream, data, (long)nrows, \
+                                   pitch, scale, tw);                                                         \
+            else                                                                                              \
+                hipLaunchKernelGGL((k_rows_c2r<T, M, K>), grid, dim3(P * K), lds, ctx->stream, data, (long)nrows, \
+                                   pitch, scale, tw);                                                         \
+        }                                                                                                     \
+        break;
+        RWCASE(1) RWCASE(2) RWCASE(4) RWCASE(8) RWCASE(16) RWCASE(32) RWCASE(64)
+#undef RWCASE
+    default:
+        throw Error(MFIC_ERR_SIZE, "bad row tile");
+    }
+    MFIC_LAUNCH_CHECK();
+}
+
+template <typename T, bool R2C>
+static void launch_rows(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale) {
+    MFIC_CHECK(n2 % 2 == 0, MFIC_ERR_SIZE, "contiguous axis must be even");
+    const int M = n2 / 2;
+    const long pitch = n2 + 2;
+    switch (M) {
+#define X(S) case S: launch_rows_M<T, S, R2C>(ctx, data, nrows, pitch, (T)scale); break;
+        MFIC_FOR_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported contiguous-axis length " + std::to_string(n2));
+    }
+}
+
+template <typename T> void fft_rows_r2c(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale) {
+    launch_rows<T, true>(ctx, data, nrows, n2, scale);
+}
+template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale) {
+    launch_rows<T, false>(ctx, data, nrows, n2, scale);
+}
+
+static void check_sizes(const size_t n[3]) {
+    MFIC_CHECK(fft_length_supported(n[0]) && fft_length_supported(n[1]), MFIC_ERR_SIZE,
+               "unsupported FFT size (need 2^a 3^b in [4, 768])");
+    MFIC_CHECK(n[2] % 2 == 0 && fft_length_supported(n[2] / 2) && n[2] <= 1536, MFIC_ERR_SIZE,
+               "unsupported contiguous FFT size (need even, n/2 = 2^a 3^b)");
+}
+
+template <typename T> void fft_r2c(mfic_ctx ctx, T* data, const size_t n[3], double scale) {
+    check_sizes(n);
+    const long nc = (long)(n[2] / 2 + 1);
+    fft_rows_r2c<T>(ctx, data, n[0] * n[1], (int)n[2], 1.0);
+    fft_pencils<T>(ctx, data, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], true, 1.0);
+    fft_pencils<T>(ctx, data, (int)n[0], (long)n[1] * nc, nc, (int)nc, (long)n[1], true, scale);
+}
+
+template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], double scale) {
+    check_sizes(n);
+    const long nc = (long)(n[2] / 2 + 1);
+    fft_pencils<T>(ctx, data, (int)n[0], (long)n[1] * nc, nc, (int)nc, (long)n[1], false, 1.0);
+    fft_pencils<T>(ctx, data, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], false, 1.0);
+    fft_rows_c2r<T>(ctx, data, n[0] * n[1], (int)n[2], scale);
+}
+
+template void fft_r2c<float>(mfic_ctx, float*, const size_t*, double);
+template void fft_r2c<double>(mfic_ctx, double*, const size_t*, double);
+template void fft_c2r<float>(mfic_ctx, float*, const size_t*, double);
+template void fft_c2r<double>(mfic_ctx, double*, const size_t*, double);
+template void fft_rows_r2c<float>(mfic_ctx, float*, size_t, int, double);
+template void fft_rows_r2c<double>(mfic_ctx, double*, size_t, int, double);
+template void fft_rows_c2r<float>(mfic_ctx, float*, size_t, int, double);
+template void fft_rows_c2r<double>(mfic_ctx, double*, size_t, int, double);
+template void fft_pencils<float>(mfic_ctx, void*, int, long, long, int, long, bool, double);
+template void fft_pencils<double>(mfic_ctx, void*, int, long, long, int, long, bool, double);
+
+}  // namespace mfic

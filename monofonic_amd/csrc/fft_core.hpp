@@ -1,0 +1,230 @@
+// Device-side FFT building blocks for gfx950 (CDNA4).
+//
+// A 1D complex transform of length L is executed by P = L/E cooperating
+// threads; each thread keeps E elements in registers (positions q + P*e,
+// q = thread index within the transform).  A Stockham autosort stage of radix
+// R makes every thread compute E/R radix-R butterflies on its own registers;
+// the outputs are exchanged through LDS between stages.  The last stage writes
+// straight back to the thread's own positions, so loads and stores of a pass
+// touch the same addresses (one global read + one global write per element).
+//
+// Radix plans are compile-time per length (all lengths 2^a 3^b up to 768);
+// every radix of a plan divides E.  Inter-stage twiddles come from a table
+// computed on the host in long double (rounded once to the working type);
+// intra-butterfly constants are 48th roots of unity.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mfic {
+
+template <typename T> struct alignas(2 * sizeof(T)) cx { T x, y; };
+
+template <typename T> __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> b) {  // a * conj(b)
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T> __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool FWD> __device__ __forceinline__ cx<T> rotq(cx<T> a) {
+    return FWD ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x};
+}
+
+// cos(2 pi a / 48), a = 0..12
+__device__ constexpr double kCos48[13] = {
+    1.0, 0.9914448613738104111446, 0.9659258262890682867497, 0.9238795325112867561282,
+    0.8660254037844386467637, 0.7933533402912351645798, 0.7071067811865475244008,
+    0.6087614290087206394161, 0.5, 0.3826834323650897717285, 0.2588190451025207623489,
+    0.1305261922200515915484, 0.0};
+
+__device__ __forceinline__ constexpr double cos48(int a) {
+    a = ((a % 48) + 48) % 48;
+    if (a <= 12) return kCos48[a];
+    if (a <= 24) return -kCos48[24 - a];
+    if (a <= 36) return -kCos48[a - 24];
+    return kCos48[48 - a];
+}
+__device__ __forceinline__ constexpr double sin48(int a) { return cos48(a - 12); }
+
+// multiply by exp(-+ 2 pi i m / R) (sign by direction), R | 48
+template <typename T, int R, bool FWD>
+__device__ __forceinline__ cx<T> rot_const(cx<T> v, int m) {
+    const int a = (m * (48 / R)) % 48;
+    if (a == 0) return v;
+    if (a == 24) return {-v.x, -v.y};
+    if (a == 12) return rotq<T, FWD>(v);
+    if (a == 36) return rotq<T, !FWD>(v);
+    const T c = (T)cos48(a);
+    const T s = FWD ? -(T)sin48(a) : (T)sin48(a);
+    return {v.x * c - v.y * s, v.x * s + v.y * c};
+}
+
+// ------------------------------------------------------------ radix DFTs --
+template <typename T, int R, bool FWD> struct Dft;
+
+template <typename T, bool FWD> struct Dft<T, 2, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) {
+        cx<T> t = a[0];
+        a[0] = cadd(t, a[1]);
+        a[1] = csub(t, a[1]);
+    }
+};
+template <typename T, bool FWD> struct Dft<T, 3, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) {
+        const T s3 = (T)0.8660254037844386467637;
+        cx<T> t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
+        cx<T> m = {a[0].x - (T)0.5 * t1.x, a[0].y - (T)0.5 * t1.y};
+        cx<T> n = rotq<T, FWD>(cscale(t2, s3));
+        a[0] = cadd(a[0], t1);
+        a[1] = cadd(m, n);
+        a[2] = csub(m, n);
+    }
+};
+template <typename T, bool FWD> struct Dft<T, 4, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) {
+        cx<T> t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+        cx<T> t2 = cadd(a[1], a[3]), t3 = rotq<T, FWD>(csub(a[1], a[3]));
+        a[0] = cadd(t0, t2);
+        a[2] = csub(t0, t2);
+        a[1] = cadd(t1, t3);
+        a[3] = csub(t1, t3);
+    }
+};
+// composite R = R1 * R2 (decimation in time: n = n2 + R2 n1, k = k1 + R1 k2)
+template <typename T, int R1, int R2, bool FWD>
+__device__ __forceinline__ void dft_composite(cx<T>* a) {
+    constexpr int R = R1 * R2;
+    cx<T> y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+        cx<T> t[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) t[n1] = a[n2 + R2 * n1];
+        Dft<T, R1, FWD>::run(t);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) y[n2 * R1 + k1] = rot_const<T, R, FWD>(t[k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        cx<T> t[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) t[n2] = y[n2 * R1 + k1];
+        Dft<T, R2, FWD>::run(t);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) a[k1 + R1 * k2] = t[k2];
+    }
+}
+template <typename T, bool FWD> struct Dft<T, 6, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) { dft_composite<T, 3, 2, FWD>(a); }
+};
+template <typename T, bool FWD> struct Dft<T, 8, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) { dft_composite<T, 2, 4, FWD>(a); }
+};
+template <typename T, bool FWD> struct Dft<T, 12, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) { dft_composite<T, 4, 3, FWD>(a); }
+};
+template <typename T, bool FWD> struct Dft<T, 16, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) { dft_composite<T, 4, 4, FWD>(a); }
+};
+
+// ------------------------------------------------------------ size plans --
+// radices r0 r1 r2 r3 (0 = unused), E = elements per thread
+template <int L> struct Plan;
+#define MFIC_PLAN(L_, E_, S_, A, B, C, D)                                 \
+    template <> struct Plan<L_> {                                          \
+        static constexpr int L = L_, E = E_, S = S_, P = L_ / E_;          \
+        static constexpr int r(int s) { return s == 0 ? A : s == 1 ? B : s == 2 ? C : D; } \
+    };
+MFIC_PLAN(4, 4, 1, 4, 0, 0, 0)
+MFIC_PLAN(6, 6, 1, 6, 0, 0, 0)
+MFIC_PLAN(8, 8, 1, 8, 0, 0, 0)
+MFIC_PLAN(12, 12, 1, 12, 0, 0, 0)
+MFIC_PLAN(16, 16, 1, 16, 0, 0, 0)
+MFIC_PLAN(24, 12, 2, 2, 12, 0, 0)
+MFIC_PLAN(32, 8, 2, 4, 8, 0, 0)
+MFIC_PLAN(48, 12, 2, 4, 12, 0, 0)
+MFIC_PLAN(64, 8, 2, 8, 8, 0, 0)
+MFIC_PLAN(72, 12, 2, 6, 12, 0, 0)
+MFIC_PLAN(96, 12, 3, 4, 4, 6, 0)
+MFIC_PLAN(128, 16, 2, 8, 16, 0, 0)
+MFIC_PLAN(144, 12, 2, 12, 12, 0, 0)
+MFIC_PLAN(192, 12, 3, 4, 4, 12, 0)
+MFIC_PLAN(256, 16, 2, 16, 16, 0, 0)
+MFIC_PLAN(288, 12, 3, 2, 12, 12, 0)
+MFIC_PLAN(384, 24, 3, 8, 12, 4, 0)
+MFIC_PLAN(512, 16, 3, 16, 8, 4, 0)
+MFIC_PLAN(576, 24, 3, 8, 6, 12, 0)
+MFIC_PLAN(768, 24, 3, 12, 8, 8, 0)
+#undef MFIC_PLAN
+
+template <int L> constexpr int plan_ns(int s) {  // span before stage s
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= Plan<L>::r(i);
+    return ns;
+}
+template <int L> constexpr int plan_tw_offset(int s) {  // twiddle table offset
+    int off = 0;
+    for (int i = 0; i < s; ++i) off += plan_ns<L>(i) * (Plan<L>::r(i) - 1);
+    return off;
+}
+template <int L> constexpr int plan_tw_size() { return plan_tw_offset<L>(Plan<L>::S); }
+
+// LDS addressing for a thread's transform: address(pos) = base + pos * stride
+struct LdsMap {
+    int base, stride;
+    __device__ __forceinline__ int at(int pos) const { return base + pos * stride; }
+};
+
+// one Stockham stage on registers v[E]; exchanges through LDS unless last
+template <typename T, int L, int S, bool FWD>
+__device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, LdsMap m, const cx<T>* __restrict__ tw) {
+    using PL = Plan<L>;
+    constexpr int E = PL::E, P = PL::P, R = PL::r(S), B = E / R, NS = plan_ns<L>(S);
+    constexpr bool LAST = (S == PL::S - 1);
+    const cx<T>* tws = tw + plan_tw_offset<L>(S);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = q + P * b;
+        const int k = j % NS;
+        cx<T> a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = v[b + r * B];
+        if (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cx<T> w = tws[k * (R - 1) + (r - 1)];
+                a[r] = FWD ? cmul(a[r], w) : cmulc(a[r], w);
+            }
+        }
+        Dft<T, R, FWD>::run(a);
+        if (LAST) {  // outputs land on this thread's own positions
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b + r * B] = a[r];
+        } else {
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[m.at(base + r * NS)] = a[r];
+        }
+    }
+    if (!LAST) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = lds[m.at(q + P * e)];
+        __syncthreads();
+    }
+}
+
+// full transform of length L on registers v[E] (positions q + P e)
+template <typename T, int L, bool FWD>
+__device__ __forceinline__ void fft_regs(cx<T>* v, int q, cx<T>* lds, LdsMap m, const cx<T>* __restrict__ tw) {
+    constexpr int S = Plan<L>::S;
+    fft_stage<T, L, 0, FWD>(v, q, lds, m, tw);
+    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD>(v, q, lds, m, tw);
+    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD>(v, q, lds, m, tw);
+    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD>(v, q, lds, m, tw);
+}
+
+}  // namespace mfic

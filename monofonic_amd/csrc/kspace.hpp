@@ -1,0 +1,14 @@
+// Host launchers of the pointwise grid kernels (kspace.hip).
+#pragma once
+#include "cosmology.hpp"
+#include "mfic_internal.hpp"
+
+namespace mfic {
+void grid_scale(mfic_grid g, double s);
+void grid_fill(mfic_grid g, double v);
+void grid_zero_dc(mfic_grid g);
+void grid_inverse_laplacian(mfic_grid g);
+void grid_phase_shift(mfic_grid g, const double s[3]);
+void convert_buffer(mfic_ctx ctx, const void* in, int in_prec, void* out, int out_prec, size_t n);
+void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double volfac, int fixing, int inversion);
+}  // namespace mfic

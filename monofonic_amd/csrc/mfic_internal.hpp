@@ -1,0 +1,95 @@
+// Internal runtime of the mfic library: contexts, device grids, errors and
+// the twiddle-table cache.  Host C++ (compiled by hipcc), not exported.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mfic.h"
+
+namespace mfic {
+
+// ---------------------------------------------------------------- errors --
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void set_last_error(const std::string& msg);
+
+#define MFIC_HIP(call)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::mfic::Error(MFIC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define MFIC_CHECK(cond, code, msg)                  \
+    do {                                             \
+        if (!(cond)) throw ::mfic::Error(code, msg); \
+    } while (0)
+#define MFIC_LAUNCH_CHECK() MFIC_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- tables --
+// device copy of the twiddle table for one transform length and precision
+struct TwTable {
+    void* dev = nullptr;
+    size_t count = 0;
+};
+
+}  // namespace mfic
+
+// ------------------------------------------------------------- handles ----
+struct mfic_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // (length, kind [0 = complex, 1 = real-row], precision) -> table
+    std::map<std::tuple<int, int, int>, mfic::TwTable> tables;
+    std::mutex mu;
+    // scratch buffers owned by the context (reused across calls)
+    std::map<std::string, std::pair<void*, size_t>> scratch;
+};
+
+struct mfic_grid_s {
+    mfic_ctx ctx = nullptr;
+    size_t n[3] = {0, 0, 0};
+    double L[3] = {0, 0, 0};
+    int prec = MFIC_F32;
+    int space = MFIC_RSPACE;
+    void* d = nullptr;
+    size_t nreal = 0;  // n0*n1*(n2+2) reals
+    bool owns = true;
+    size_t elem_size() const { return prec == MFIC_F64 ? 8 : 4; }
+    size_t bytes() const { return nreal * elem_size(); }
+    size_t nc() const { return n[2] / 2 + 1; }
+};
+
+namespace mfic {
+
+// scratch buffer of at least `bytes`, keyed by name, owned by the context
+void* ctx_scratch(mfic_ctx ctx, const std::string& key, size_t bytes);
+
+// twiddles: complex transform of length L (kind 0), or real rows of length
+// 2M (kind 1: complex length M table followed by M+1 untangle factors)
+const void* twiddles(mfic_ctx ctx, int L, int kind, int prec);
+
+// FFT entry points on raw device buffers of layout [n0][n1][n2+2]
+template <typename T> void fft_r2c(mfic_ctx ctx, T* data, const size_t n[3], double scale);
+template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], double scale);
+bool fft_length_supported(size_t n);
+
+// pass-level entry points used by fused pipelines
+template <typename T> void fft_rows_r2c(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
+template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
+// complex pencils of length L: element (o, p, t) at data + o*os + p*ps + t,
+// t < tlen, o < nouter (offsets in complex elements)
+template <typename T>
+void fft_pencils(mfic_ctx ctx, void* data, int L, long ps, long os, int tlen, long nouter, bool fwd,
+                 double scale);
+
+}  // namespace mfic

@@ -1,0 +1,193 @@
+/*
+ * CPU FP64 oracle for the monofonIC initial-condition hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library (monofonic_amd/)
+ * links, loads or calls this code; only tests/, the smoke check of
+ * __graft_entry__.py and the cpu_baseline leg of bench.py use it, and only
+ * as the checker or the timed CPU baseline.
+ *
+ * Each function restates the reference algorithm it cites (paths relative to
+ * the monofonIC source tree).  Parity status per function is recorded next to
+ * each declaration:  "pinned: <how>"  or  "parity unpinned: <why>".
+ *
+ * Data layout (grid_fft.cc:45, 51-54): a field of logical size n0 x n1 x n2 is
+ * stored as double[n0][n1][n2+2]; its k-space view is complex
+ * double[n0][n1][n2/2+1][2] in the same buffer.
+ */
+#ifndef MFIC_ORACLE_H
+#define MFIC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct orc_plt;
+
+/* ------------------------------------------------------------------ FFT --- */
+/* Unitary in-place 3D real<->complex transforms, FFTW sign convention
+ * (forward e^{-ikx}); both directions scale by 1/sqrt(n0 n1 n2)
+ * (grid_fft.cc:69, 199-205, 207-270).  c2r transforms axes 0 and 1 first and
+ * the contiguous axis last, discarding the imaginary part of the kz=0 and
+ * kz=n2/2 terms (the FFTW / numpy irfftn convention).
+ * pinned: numpy.fft.rfftn/irfftn(norm="ortho") fixtures + identities. */
+void orc_fft3d_r2c(double *data, int n0, int n1, int n2);
+void orc_fft3d_c2r(double *data, int n0, int n1, int n2);
+/* 1D complex FFT of length n (interleaved re/im), sign -1 forward, +1 back,
+ * unnormalised.  pinned: numpy.fft.fft fixtures. */
+void orc_fft1d(double *z, int n, int sign);
+
+/* ------------------------------------------------- correctly rounded math - */
+/* log(x) and sin/cos(x) evaluated in double-double and rounded once, so the
+ * same IEEE operation sequence on host and device gives identical results.
+ * pinned: agreement with glibc log/sin/cos and Python decimal high-precision
+ * evaluation on random and edge inputs. */
+double orc_cr_log(double x);
+void orc_cr_sincos(double x, double *s, double *c);
+
+/* -------------------------------------------------------------- RANLUX --- */
+/* Lüscher's double-precision RANLUX (ranlxd) with GSL's gsl_rng_set seeding,
+ * as used through gsl_rng_ranlxd1 by random_ngenic.cc:44-45, 104-114.
+ * luxury = 202 (ranlxd1) or 397 (ranlxd2).  State words are 48-bit integers.
+ * pinned: GSL rng test-suite values (ranlxd1 / ranlxd2, seed 1, 10000th
+ * gsl_rng_get) -- see tests/test_oracle_rng.py. */
+typedef struct {
+    uint64_t x[12];
+    int carry;
+    int ir, jr, ir_old, pr;
+} orc_ranlxd;
+void orc_ranlxd_set(orc_ranlxd *st, unsigned long seed, int luxury);
+uint64_t orc_ranlxd_next48(orc_ranlxd *st);      /* next 48-bit word      */
+double orc_ranlxd_uniform(orc_ranlxd *st);       /* gsl_rng_uniform       */
+unsigned long orc_ranlxd_get(orc_ranlxd *st);    /* gsl_rng_get (32 bit)  */
+
+/* ------------------------------------------------------------- NGenIC --- */
+/* Seed table (random_ngenic.cc:39-75) and k-space white noise
+ * (random_ngenic.cc:84-154), non-distributed branch.  `kdata` is an
+ * n x n x (n+2) double buffer receiving the k-space field.
+ * Stream values: parity unpinned against GSL itself (GSL absent); pinned by the
+ * RANLUX test values above, structural invariants and statistics. */
+void orc_ngenic_seedtable(long seed, size_t n, uint32_t *table);
+void orc_ngenic_fill(const uint32_t *table, size_t n, double *kdata);
+
+/* ----------------------------------------------------------- cosmology --- */
+typedef struct {
+    /* inputs (cosmology_parameters.hh:84-149 keys) */
+    double Omega_m, Omega_b, H0, n_s, sigma_8, A_s, k_p;   /* A_s <= -1: sigma_8 normalisation */
+    double Tcmb, N_ur, m_nu[3], deg_nu[3], O_nu_norm;
+    double w_0, w_a, Omega_k, alpha_s, beta_s;
+    int zero_radiation;
+    double zstart, ztarget;
+    /* derived (cosmology_parameters.hh:177-216, cosmology_calculator.hh) */
+    double h, Omega_nu_massive, Omega_gamma, Omega_r, Omega_c, Omega_DE;
+    double Dnow, Dplus_start, Dplus_target, vfac_start, f_start;
+    double sigma8_measured, pnorm, sqrtpnorm;
+    /* Eisenstein & Hu scalars (transfer_eisenstein.cc:58-122) */
+    double eh[24];
+    /* growth table (cosmology_calculator.hh:92-152) */
+    int ntab;
+    double *tab_loga, *tab_logD, *tab_logf, *spl_D, *spl_f;
+} orc_cosmo;
+
+/* Planck2018EE+BAO+SN defaults for every key (cosmology_parameters.cc:110-133,
+ * cosmology_parameters.hh:84-149 with ParameterSet=none). */
+void orc_cosmo_defaults(orc_cosmo *c);
+/* derived parameters, growth, sigma_8 normalisation. returns 0 on success */
+int orc_cosmo_init(orc_cosmo *c);
+void orc_cosmo_free(orc_cosmo *c);
+double orc_cosmo_H(const orc_cosmo *c, double a);
+double orc_cosmo_D(const orc_cosmo *c, double a);        /* D(a)/Dnow     */
+double orc_cosmo_f(const orc_cosmo *c, double a);
+double orc_cosmo_vfact(const orc_cosmo *c, double a);
+double orc_tk_eh(const orc_cosmo *c, double k_hmpc);      /* E&H total T(k) */
+double orc_amplitude(const orc_cosmo *c, double k_hmpc);  /* get_amplitude */
+/* adaptive Gauss-Kronrod integral of the sigma_8 integrand over [a,b] */
+double orc_integrate_sigma8(const orc_cosmo *c, double a, double b, double rel);
+/* raw growth table (a, D, f) for checks; returns count */
+int orc_cosmo_growth_table(const orc_cosmo *c, double *a, double *D, double *f, int max);
+
+/* ----------------------------------------------------------------- LPT --- */
+enum { ORC_OP_ASSIGN = 0, ORC_OP_ADD = 1, ORC_OP_SUB = 2, ORC_OP_MADD = 3 };
+enum { ORC_KIND_HESS = 0, ORC_KIND_SUM_HESS = 1, ORC_KIND_DIFF_HESS = 2,
+       ORC_KIND_GRAD = 3, ORC_KIND_IDENT = 4 };
+typedef struct {
+    const double *f;   /* k-space N-grid field            */
+    int kind;
+    int d[4];          /* directions: HESS uses d[0],d[1]; SUM/DIFF all four */
+} orc_operand;
+
+/* OrszagConvolver::convolve2 (convolution.hh:495-518) on an N^3 grid of box
+ * length L, with the operand builders of convolution.hh:74-275 and the output
+ * operators of operators.hh:25-35.  pinned: brute-force O(N^6) dealiased
+ * convolution on 8^3 and 12^3 grids, closed forms. */
+void orc_convolve2(int n, double L, const orc_operand *a, const orc_operand *b,
+                   double *out, int op, double coef);
+/* convolve3 = trunc(trunc(a b) c) (convolution.hh:525-532) */
+void orc_convolve3(int n, double L, const orc_operand *a, const orc_operand *b,
+                   const orc_operand *c, double *out, int op, double coef);
+
+typedef struct {
+    int n;                 /* GridRes                                  */
+    double L;              /* BoxLength [Mpc/h]                        */
+    int lpt_order;         /* 1..3                                     */
+    int lattice;           /* 0 = sc, 1 = bcc                          */
+    int plt;               /* PLT gradient / velocity correction       */
+    int do_fixing, do_inversion;
+    long seed;
+    double lunit, vunit;   /* output plugin position/velocity unit     */
+} orc_ic_cfg;
+
+/* White noise -> phi1, phi2, phi3, A3 (ic_generator.cc:288-658), growth
+ * factors applied.  Each output is an n x n x (n+2) k-space buffer; phi2/phi3/
+ * A3 may be NULL when lpt_order is too low.  Also returns the scalars used. */
+typedef struct {
+    double Dplus0, vfac, g1, g2, g3, g3c, vfac1, vfac2, vfac3, volfac;
+} orc_lpt_coef;
+void orc_lpt_coefs(const orc_ic_cfg *cfg, const orc_cosmo *c, orc_lpt_coef *k);
+void orc_lpt_potentials(const orc_ic_cfg *cfg, const orc_cosmo *c, const double *wnoise_k,
+                        double *phi, double *phi2, double *phi3, double *A3x,
+                        double *A3y, double *A3z);
+
+/* One displacement (comp 0..2) or velocity (comp 3..5) component in k-space
+ * after unit scaling and DC zeroing (ic_generator.cc:931-970, 987-1047). */
+void orc_lpt_component_k(const orc_ic_cfg *cfg, const orc_lpt_coef *k,
+                         const struct orc_plt *plt,
+                         const double *phi, const double *phi2, const double *phi3,
+                         const double *A3x, const double *A3y, const double *A3z,
+                         int comp, double *out_k);
+
+/* Bravais readout: real-space component field -> particle values.
+ * lattice_generator::set_positions / set_velocities (particle_generator.hh:
+ * 499-544, 597-639) for sc / bcc, lattice_index 0.  `field` is the real-space
+ * component (A1 layout) and is modified (phase shifted) for bcc.
+ * out has 3*n^3*overload doubles interleaved [ipart][3]. */
+void orc_lattice_set_positions(int n, double L, int lattice, int idim, double lunit,
+                               double *field, double *pos);
+void orc_lattice_set_velocities(int n, double L, int lattice, int idim,
+                                double *field, double *vel);
+/* shift_field (grid_fft.hh:832-842): r-space in, r-space out */
+void orc_shift_field(int n, double L, const double s[3], double *field);
+
+/* Whole run, generic (field_lagrangian) output: 6 fields dx,dy,dz,vx,vy,vz,
+ * each an n x n x (n+2) real buffer, or particle output when pos != NULL. */
+int orc_ic_run(const orc_ic_cfg *cfg, orc_cosmo *c, double *fields6,
+               double *pos, double *vel, double stage_seconds[8]);
+
+/* ----------------------------------------------------------------- PLT --- */
+/* particle::lattice_gradient (particle_plt.hh:50-587) tables on an ng^3 mesh */
+typedef struct orc_plt {
+    int ng;
+    double *Dr, *Dtheta, *Dphi, *vfac;   /* ng x ng x (ng/2+1) real parts */
+} orc_plt;
+int orc_plt_init(orc_plt *p, int lattice, int ng);
+void orc_plt_free(orc_plt *p);
+/* gradient(idim, ijk) imaginary part and vfac_corr(ijk) for an N-grid */
+double orc_plt_gradient(const orc_plt *p, int idim, int n, double L, int i, int j, int k);
+double orc_plt_vfac_corr(const orc_plt *p, int n, int i, int j, int k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
