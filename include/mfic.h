@@ -245,7 +245,8 @@ int mfic_run_execute(mfic_run r, int blocking);
 int mfic_run_field(mfic_run r, int comp, mfic_grid *out);
 /* particle arrays (device pointers, 3*npart doubles each) when particles=1 */
 int mfic_run_particles(mfic_run r, double **pos, double **vel, size_t *npart);
-/* potentials after growth scaling (phi1, phi2, phi3, A3x, A3y, A3z) */
+/* k-space potentials phi1, phi2, phi3, A3x, A3y, A3z (which = 0..5), before
+ * growth scaling: the assembly applies g1..g3c (ic_generator.cc:644-658) */
 int mfic_run_potential(mfic_run r, int which, mfic_grid *out);
 
 #ifdef __cplusplus

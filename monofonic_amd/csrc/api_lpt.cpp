@@ -1,0 +1,420 @@
+// C ABI, second half: white noise, convolver, LPT sources and assembly, PLT
+// tables, lattice readout and the whole-run object.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cosmology.hpp"
+#include "kspace.hpp"
+#include "lpt.hpp"
+#include "mfic_internal.hpp"
+#include "ngenic.hpp"
+#include "orszag.hpp"
+
+namespace mfic {
+template <typename F> int guard_call(F&& f);
+}
+
+struct mfic_conv_s {
+    std::unique_ptr<mfic::ConvCore> core;
+    void *f1p = nullptr, *f2p = nullptr, *fbuf = nullptr;
+    ~mfic_conv_s() {
+        if (f1p) (void)hipFree(f1p);
+        if (f2p) (void)hipFree(f2p);
+        if (fbuf) (void)hipFree(fbuf);
+    }
+};
+
+struct mfic_plt_s {
+    mfic::PltTables t;
+};
+
+struct mfic_run_s {
+    mfic_ctx ctx = nullptr;
+    mfic_ic_config cfg{};
+    mfic_cosmo_params cosmo{};
+    mfic_cosmo_scalars sc{};
+    mfic::EhParams eh{};
+    mfic_lpt_coef coef{};
+    mfic_grid pot[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // phi1 phi2 phi3 A3x A3y A3z
+    mfic_grid tmp = nullptr;
+    mfic_grid fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t* d_table = nullptr;
+    double *d_pos = nullptr, *d_vel = nullptr;
+    size_t npart = 0;
+    mfic::PltTables plt;
+    bool has_plt = false;
+    ~mfic_run_s();
+};
+
+namespace mfic {
+
+
+template <typename F> int guard_call(F&& f) {
+    try {
+        f();
+        return MFIC_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        set_last_error(e.what());
+        return MFIC_ERR_CONFIG;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return MFIC_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MFIC_ERR_ARG;
+    }
+}
+
+static void need(mfic_grid g, const char* what) {
+    MFIC_CHECK(g && g->d, MFIC_ERR_ARG, std::string("null or unallocated grid: ") + what);
+}
+
+static OperandDesc to_desc(const mfic_operand* o) {
+    MFIC_CHECK(o && o->f && o->f->d, MFIC_ERR_ARG, "null operand");
+    MFIC_CHECK(o->kind >= MFIC_OPND_HESS && o->kind <= MFIC_OPND_IDENT, MFIC_ERR_ARG, "bad operand kind");
+    for (int i = 0; i < 4; ++i) MFIC_CHECK(o->d[i] >= 0 && o->d[i] < 3, MFIC_ERR_ARG, "operand direction out of range");
+    MFIC_CHECK(o->f->space == MFIC_KSPACE, MFIC_ERR_STATE, "operand fields must be in k-space");
+    return OperandDesc{o->f->d, o->kind, {o->d[0], o->d[1], o->d[2], o->d[3]}};
+}
+
+static void* dmalloc(size_t bytes, const char* what) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) throw Error(MFIC_ERR_ALLOC, std::string("device allocation failed: ") + what);
+    return p;
+}
+
+static mfic_grid new_grid(mfic_ctx ctx, size_t n, double L, int prec, int space) {
+    const size_t nn[3] = {n, n, n};
+    const double LL[3] = {L, L, L};
+    mfic_grid g = nullptr;
+    int rc = mfic_grid_create(ctx, nn, LL, prec, space, &g);
+    if (rc != MFIC_OK) throw Error(rc, mfic_last_error());
+    return g;
+}
+
+static void compute_coefs(const mfic_cosmo_params& p, size_t n, double L, int order, const mfic_cosmo_scalars& s,
+                          mfic_lpt_coef* k) {
+    k->volfac = std::pow(L / (double)n / 2.0 / 3.14159265358979323846, 1.5);
+    k->Dplus0 = s.Dplus_start;
+    k->vfac = s.vfac_start;
+    const double D = k->Dplus0;
+    k->g1 = -D;
+    k->g2 = order > 1 ? -3.0 / 7.0 * D * D : 0.0;
+    k->g3 = order > 2 ? 1.0 / 3.0 * D * D * D : 0.0;
+    k->g3c = order > 2 ? 1.0 / 7.0 * D * D * D : 0.0;
+    k->vfac1 = k->vfac;
+    k->vfac2 = 2 * k->vfac;
+    k->vfac3 = 3 * k->vfac;
+}
+
+static void component_coefs(const mfic_lpt_coef& k, int comp, double cf[4]) {
+    if (comp < 3) {
+        cf[0] = k.g1; cf[1] = k.g2; cf[2] = k.g3; cf[3] = k.g3c;
+    } else {
+        cf[0] = k.vfac1 * k.g1; cf[1] = k.vfac2 * k.g2; cf[2] = k.vfac3 * k.g3; cf[3] = k.vfac3 * k.g3c;
+    }
+}
+
+}  // namespace mfic
+
+using namespace mfic;
+
+mfic_run_s::~mfic_run_s() {
+    for (auto& g : pot) if (g) mfic_grid_destroy(g);
+    for (auto& g : fields) if (g) mfic_grid_destroy(g);
+    if (tmp) mfic_grid_destroy(tmp);
+    if (d_table) (void)hipFree(d_table);
+    if (d_pos) (void)hipFree(d_pos);
+    if (d_vel) (void)hipFree(d_vel);
+}
+
+extern "C" {
+
+int mfic_ngenic_seedtable(long seed, size_t n, uint32_t* table) {
+    return guard_call([&] {
+        MFIC_CHECK(table && n > 0, MFIC_ERR_ARG, "null table or empty grid");
+        ngenic_seedtable(seed, n, table);
+    });
+}
+
+int mfic_ngenic_fill(mfic_grid g, long seed) {
+    return guard_call([&] {
+        need(g, "noise");
+        const size_t n = g->n[0];
+        std::vector<uint32_t> tab(n * n);
+        ngenic_seedtable(seed, n, tab.data());
+        uint32_t* dt = static_cast<uint32_t*>(ctx_scratch(g->ctx, "seedtable", n * n * sizeof(uint32_t)));
+        MFIC_HIP(hipMemcpyAsync(dt, tab.data(), n * n * sizeof(uint32_t), hipMemcpyHostToDevice, g->ctx->stream));
+        grid_fill(g, 0.0);
+        ngenic_fill(g, dt);
+        g->space = MFIC_KSPACE;
+        MFIC_HIP(hipStreamSynchronize(g->ctx->stream));
+    });
+}
+
+int mfic_conv_create(mfic_ctx ctx, const size_t n[3], const double L[3], int prec, mfic_conv* out) {
+    return guard_call([&] {
+        MFIC_CHECK(ctx && n && L && out, MFIC_ERR_ARG, "null argument");
+        auto c = std::make_unique<mfic_conv_s>();
+        c->core = std::make_unique<ConvCore>(ctx, n, L, prec);
+        const size_t pb = c->core->padded_reals() * c->core->esize();
+        c->f1p = dmalloc(pb, "convolver padded buffer");
+        c->f2p = dmalloc(pb, "convolver padded buffer");
+        c->fbuf = dmalloc(c->core->grid_reals() * c->core->esize(), "convolver N-grid buffer");
+        *out = c.release();
+    });
+}
+
+int mfic_conv_destroy(mfic_conv c) {
+    return guard_call([&] {
+        if (!c) return;
+        (void)hipStreamSynchronize(c->core->ctx->stream);
+        delete c;
+    });
+}
+
+static void conv_check_out(mfic_conv c, mfic_grid out) {
+    MFIC_CHECK(c, MFIC_ERR_ARG, "null convolver");
+    need(out, "output");
+    MFIC_CHECK(out->prec == c->core->prec && out->n[0] == c->core->N[0] && out->n[1] == c->core->N[1] &&
+                   out->n[2] == c->core->N[2],
+               MFIC_ERR_ARG, "output grid does not match the convolver");
+}
+
+int mfic_conv_convolve2(mfic_conv c, const mfic_operand* a, const mfic_operand* b, mfic_grid out, int op, double coef) {
+    return guard_call([&] {
+        conv_check_out(c, out);
+        MFIC_CHECK(op >= MFIC_OP_ASSIGN && op <= MFIC_OP_MADD, MFIC_ERR_ARG, "bad output operator");
+        const OperandDesc da = to_desc(a), db = to_desc(b);
+        MFIC_CHECK(a->f->prec == c->core->prec && b->f->prec == c->core->prec, MFIC_ERR_ARG, "precision mismatch");
+        c->core->convolve2(da, db, out->d, op, coef, c->f1p, c->f2p);
+        out->space = MFIC_KSPACE;
+    });
+}
+
+int mfic_conv_convolve3(mfic_conv c, const mfic_operand* a, const mfic_operand* b, const mfic_operand* cc,
+                        mfic_grid out, int op, double coef) {
+    return guard_call([&] {
+        conv_check_out(c, out);
+        MFIC_CHECK(op >= MFIC_OP_ASSIGN && op <= MFIC_OP_MADD, MFIC_ERR_ARG, "bad output operator");
+        const OperandDesc da = to_desc(a), db = to_desc(b), dc = to_desc(cc);
+        c->core->convolve3(da, db, dc, out->d, op, coef, c->f1p, c->f2p, c->fbuf);
+        out->space = MFIC_KSPACE;
+    });
+}
+
+int mfic_lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_grid phi3, mfic_grid A3x,
+                     mfic_grid A3y, mfic_grid A3z) {
+    return guard_call([&] {
+        MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
+        MFIC_CHECK(order >= 1 && order <= 3, MFIC_ERR_CONFIG, "LPT order must be 1, 2 or 3");
+        need(phi1, "phi1");
+        MFIC_CHECK(phi1->space == MFIC_KSPACE, MFIC_ERR_STATE, "phi1 must be in k-space");
+        if (order > 1) need(phi2, "phi2");
+        if (order > 2) { need(phi3, "phi3"); need(A3x, "A3x"); need(A3y, "A3y"); need(A3z, "A3z"); }
+        lpt_sources(ctx, order, phi1, phi2, phi3, A3x, A3y, A3z);
+    });
+}
+
+int mfic_lpt_coefficients(const mfic_cosmo_params* p, size_t n, double L, int order, mfic_lpt_coef* out) {
+    return guard_call([&] {
+        MFIC_CHECK(p && out && n > 0 && L > 0, MFIC_ERR_ARG, "bad argument");
+        mfic_cosmo_scalars s;
+        cosmo_compute(*p, &s, nullptr);
+        compute_coefs(*p, n, L, order, s, out);
+    });
+}
+
+int mfic_plt_create(mfic_ctx ctx, int lattice, int ng, mfic_plt* out) {
+    return guard_call([&] {
+        MFIC_CHECK(ctx && out, MFIC_ERR_ARG, "null argument");
+        auto p = std::make_unique<mfic_plt_s>();
+        plt_build(ctx, lattice, ng, p->t);
+        *out = p.release();
+    });
+}
+
+int mfic_plt_destroy(mfic_plt p) {
+    return guard_call([&] { delete p; });
+}
+
+int mfic_plt_download(mfic_plt p, double* Dr, double* Dtheta, double* Dphi, double* vfac) {
+    return guard_call([&] {
+        MFIC_CHECK(p, MFIC_ERR_ARG, "null tables");
+        const size_t nk = (size_t)p->t.ng * p->t.ng * (p->t.ng / 2 + 1);
+        double* dst[4] = {Dr, Dtheta, Dphi, vfac};
+        for (int c = 0; c < 4; ++c)
+            if (dst[c]) std::memcpy(dst[c], p->t.host.data() + c * nk, nk * sizeof(double));
+    });
+}
+
+int mfic_lpt_component(mfic_ctx ctx, int order, int comp, const mfic_lpt_coef* k, mfic_plt plt, double unit,
+                       mfic_grid phi1, mfic_grid phi2, mfic_grid phi3, mfic_grid A3x, mfic_grid A3y, mfic_grid A3z,
+                       mfic_grid out) {
+    return guard_call([&] {
+        MFIC_CHECK(ctx && k, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(comp >= 0 && comp < 6, MFIC_ERR_ARG, "component must be 0..5");
+        MFIC_CHECK(order >= 1 && order <= 3, MFIC_ERR_CONFIG, "LPT order must be 1, 2 or 3");
+        need(phi1, "phi1");
+        need(out, "out");
+        const void* phi[3] = {phi1->d, order > 1 ? (need(phi2, "phi2"), phi2->d) : nullptr,
+                              order > 2 ? (need(phi3, "phi3"), phi3->d) : nullptr};
+        const void* A3[3] = {nullptr, nullptr, nullptr};
+        if (order > 2) {
+            need(A3x, "A3x"); need(A3y, "A3y"); need(A3z, "A3z");
+            A3[0] = A3x->d; A3[1] = A3y->d; A3[2] = A3z->d;
+        }
+        double cf[4];
+        component_coefs(*k, comp, cf);
+        PltDev pv;
+        if (plt) pv = plt->t.view;
+        MFIC_CHECK(phi1->n[0] == phi1->n[1] && phi1->n[1] == phi1->n[2], MFIC_ERR_SIZE, "assembly needs a cubic grid");
+        assemble_component(ctx, out->prec, out->n, out->L[0], order, comp, cf, unit, pv, phi, A3, out->d, nullptr);
+        out->space = MFIC_KSPACE;
+    });
+}
+
+int mfic_lattice_readout(mfic_grid field, int lattice, int idim, int is_pos, double lunit, double* dst) {
+    return guard_call([&] {
+        need(field, "field");
+        MFIC_CHECK(dst, MFIC_ERR_ARG, "null destination");
+        MFIC_CHECK(lattice == MFIC_LATTICE_SC || lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
+                   "readout supports sc and bcc lattices");
+        MFIC_CHECK(idim >= 0 && idim < 3, MFIC_ERR_ARG, "idim must be 0..2");
+        MFIC_CHECK(field->space == MFIC_RSPACE, MFIC_ERR_STATE, "field must be in real space");
+        const int n = (int)field->n[0];
+        lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, 0.0, 0, dst);
+        if (lattice == MFIC_LATTICE_BCC) {
+            const double s[3] = {0.5, 0.5, 0.5};
+            int rc = mfic_grid_shift(field, s, 1);
+            if (rc != MFIC_OK) throw Error(rc, mfic_last_error());
+            lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, 0.5, (size_t)n * n * n, dst);
+        }
+    });
+}
+
+int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_params* cosmo, mfic_run* out) {
+    return guard_call([&] {
+        MFIC_CHECK(ctx && cfg && cosmo && out, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(cfg->lpt_order >= 1 && cfg->lpt_order <= 3, MFIC_ERR_CONFIG, "LPTorder must be 1, 2 or 3");
+        MFIC_CHECK(cfg->lattice == MFIC_LATTICE_SC || cfg->lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
+                   "ParticleLoad must be sc or bcc");
+        MFIC_CHECK(cfg->prec == MFIC_F32 || cfg->prec == MFIC_F64, MFIC_ERR_CONFIG, "bad precision");
+        MFIC_CHECK(cfg->n % 2 == 0 || cfg->lpt_order == 1, MFIC_ERR_SIZE,
+                   "Orszag convolver for LPTorder>1 requires even grid size!");
+        MFIC_CHECK(cfg->L > 0, MFIC_ERR_CONFIG, "BoxLength must be positive");
+        auto r = std::make_unique<mfic_run_s>();
+        r->ctx = ctx;
+        r->cfg = *cfg;
+        r->cosmo = *cosmo;
+        cosmo_compute(*cosmo, &r->sc, &r->eh);
+        compute_coefs(*cosmo, cfg->n, cfg->L, cfg->lpt_order, r->sc, &r->coef);
+        const size_t n = cfg->n;
+        for (int p = 0; p < (cfg->lpt_order == 1 ? 1 : cfg->lpt_order == 2 ? 2 : 6); ++p)
+            r->pot[p] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
+        if (cfg->particles) {
+            r->tmp = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
+            r->npart = (size_t(1) << cfg->lattice) * n * n * n;
+            r->d_pos = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "particle positions"));
+            r->d_vel = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "particle velocities"));
+        } else {
+            for (int c = 0; c < 6; ++c) r->fields[c] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_RSPACE);
+        }
+        // RNG_ngenic constructor work (seed table) happens here, outside the run
+        std::vector<uint32_t> tab(n * n);
+        ngenic_seedtable(cfg->seed, n, tab.data());
+        r->d_table = static_cast<uint32_t*>(dmalloc(n * n * sizeof(uint32_t), "seed table"));
+        MFIC_HIP(hipMemcpy(r->d_table, tab.data(), n * n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (cfg->plt) {
+            plt_build(ctx, cfg->lattice, 64, r->plt);
+            r->has_plt = true;
+        }
+        MFIC_HIP(hipStreamSynchronize(ctx->stream));
+        *out = r.release();
+    });
+}
+
+int mfic_run_destroy(mfic_run r) {
+    return guard_call([&] {
+        if (!r) return;
+        (void)hipStreamSynchronize(r->ctx->stream);
+        delete r;
+    });
+}
+
+int mfic_run_execute(mfic_run r, int blocking) {
+    return guard_call([&] {
+        MFIC_CHECK(r, MFIC_ERR_ARG, "null run");
+        const mfic_ic_config& c = r->cfg;
+        mfic_grid phi1 = r->pot[0];
+        grid_fill(phi1, 0.0);
+        ngenic_fill(phi1, r->d_table);
+        phi1->space = MFIC_KSPACE;
+        phi1_from_noise(phi1, phi1, r->eh, r->coef.volfac, c.do_fixing, c.do_inversion);
+        if (c.lpt_order > 1) lpt_sources(r->ctx, c.lpt_order, phi1, r->pot[1], r->pot[2], r->pot[3], r->pot[4], r->pot[5]);
+        const void* phi[3] = {phi1->d, r->pot[1] ? r->pot[1]->d : nullptr, r->pot[2] ? r->pot[2]->d : nullptr};
+        const void* A3[3] = {r->pot[3] ? r->pot[3]->d : nullptr, r->pot[4] ? r->pot[4]->d : nullptr,
+                             r->pot[5] ? r->pot[5]->d : nullptr};
+        PltDev pv;
+        if (r->has_plt) pv = r->plt.view;
+        const size_t nn[3] = {c.n, c.n, c.n};
+        const double norm = 1.0 / std::sqrt((double)c.n * (double)c.n * (double)c.n);
+        for (int comp = 0; comp < 6; ++comp) {
+            double cf[4];
+            component_coefs(r->coef, comp, cf);
+            const double unit = comp < 3 ? c.lunit / c.L : c.vunit / c.L;
+            const int nsub = c.particles ? (1 << c.lattice) : 1;
+            for (int sub = 0; sub < nsub; ++sub) {
+                mfic_grid out = c.particles ? r->tmp : r->fields[comp];
+                const double dxs = 0.5 * c.L / (double)c.n;
+                const double shift[3] = {dxs, dxs, dxs};
+                assemble_component(r->ctx, c.prec, nn, c.L, c.lpt_order, comp, cf, unit, pv, phi, A3, out->d,
+                                   sub ? shift : nullptr);
+                if (c.prec == MFIC_F64) fft_c2r<double>(r->ctx, (double*)out->d, nn, norm);
+                else fft_c2r<float>(r->ctx, (float*)out->d, nn, norm);
+                out->space = MFIC_RSPACE;
+                if (c.particles) {
+                    double* dst = comp < 3 ? r->d_pos : r->d_vel;
+                    lattice_readout_raw(r->ctx, c.prec, out->d, (int)c.n, comp % 3, comp < 3, c.lunit,
+                                        sub ? 0.5 : 0.0, sub * c.n * c.n * c.n, dst);
+                }
+            }
+        }
+        if (blocking) MFIC_HIP(hipStreamSynchronize(r->ctx->stream));
+    });
+}
+
+int mfic_run_field(mfic_run r, int comp, mfic_grid* out) {
+    return guard_call([&] {
+        MFIC_CHECK(r && out && comp >= 0 && comp < 6, MFIC_ERR_ARG, "bad argument");
+        MFIC_CHECK(!r->cfg.particles, MFIC_ERR_STATE, "run was configured for particle output");
+        *out = r->fields[comp];
+    });
+}
+
+int mfic_run_particles(mfic_run r, double** pos, double** vel, size_t* npart) {
+    return guard_call([&] {
+        MFIC_CHECK(r, MFIC_ERR_ARG, "null run");
+        MFIC_CHECK(r->cfg.particles, MFIC_ERR_STATE, "run was configured for field output");
+        if (pos) *pos = r->d_pos;
+        if (vel) *vel = r->d_vel;
+        if (npart) *npart = r->npart;
+    });
+}
+
+int mfic_run_potential(mfic_run r, int which, mfic_grid* out) {
+    return guard_call([&] {
+        MFIC_CHECK(r && out && which >= 0 && which < 6, MFIC_ERR_ARG, "bad argument");
+        MFIC_CHECK(r->pot[which] != nullptr, MFIC_ERR_STATE, "potential not computed for this LPT order");
+        *out = r->pot[which];
+    });
+}
+
+}  // extern "C"

@@ -133,7 +133,7 @@ template <int L> static void fill_tw(std::vector<std::complex<long double>>& out
 }
 
 #define MFIC_FOR_SIZES(X) \
-    X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768)
+    X(4) X(6) X(8) X(9) X(12) X(16) X(18) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768)
 
 bool fft_length_supported(size_t n) {
     switch (n) {
@@ -293,7 +293,7 @@ template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int
 
 static void check_sizes(const size_t n[3]) {
     MFIC_CHECK(fft_length_supported(n[0]) && fft_length_supported(n[1]), MFIC_ERR_SIZE,
-               "unsupported FFT size (need 2^a 3^b in [4, 768])");
+               "unsupported FFT size (supported: 4 6 8 9 12 16 18 24 32 36 48 64 72 96 128 144 192 256 288 384 512 576 768)");
     MFIC_CHECK(n[2] % 2 == 0 && fft_length_supported(n[2] / 2) && n[2] <= 1536, MFIC_ERR_SIZE,
                "unsupported contiguous FFT size (need even, n/2 = 2^a 3^b)");
 }

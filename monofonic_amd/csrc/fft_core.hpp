@@ -141,10 +141,13 @@ template <int L> struct Plan;
 MFIC_PLAN(4, 4, 1, 4, 0, 0, 0)
 MFIC_PLAN(6, 6, 1, 6, 0, 0, 0)
 MFIC_PLAN(8, 8, 1, 8, 0, 0, 0)
+MFIC_PLAN(9, 3, 2, 3, 3, 0, 0)
 MFIC_PLAN(12, 12, 1, 12, 0, 0, 0)
 MFIC_PLAN(16, 16, 1, 16, 0, 0, 0)
+MFIC_PLAN(18, 6, 2, 3, 6, 0, 0)
 MFIC_PLAN(24, 12, 2, 2, 12, 0, 0)
 MFIC_PLAN(32, 8, 2, 4, 8, 0, 0)
+MFIC_PLAN(36, 6, 2, 6, 6, 0, 0)
 MFIC_PLAN(48, 12, 2, 4, 12, 0, 0)
 MFIC_PLAN(64, 8, 2, 8, 8, 0, 0)
 MFIC_PLAN(72, 12, 2, 6, 12, 0, 0)

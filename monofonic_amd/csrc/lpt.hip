@@ -1,0 +1,189 @@
+// LPT displacement / velocity assembly in k-space (ic_generator.cc:931-970,
+// 987-1047) and the Bravais lattice readout of q + Psi in FP64
+// (particle_generator.hh:499-544, 597-639).
+#include <cmath>
+
+#include "fft_core.hpp"
+#include "lpt.hpp"
+#include "mfic_internal.hpp"
+
+namespace mfic {
+
+static inline unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct AsmArgs {
+    const void* phi[3];  // phi1, phi2, phi3 (unscaled potentials; may be null)
+    const void* A3[3];   // A3x, A3y, A3z
+    int n0, n1, n2, nc, order, comp;
+    double kf0, kf1, kf2;
+    double c1, c2, c3, cA;  // phi_n and A3 coefficients (growth [x vfac])
+    double unit;            // position or velocity unit / L
+    double sh0, sh1, sh2;   // optional phase shift s_a dx_a (bcc sub-lattice)
+    PltDev plt;             // PLT tables (plt.Dr == nullptr: Fourier gradient)
+};
+
+// Fourier gradient (operators.hh:50-55): i (idx - [idx>n/2] n) 2pi/L, 0 at n/2
+__device__ __forceinline__ double fgrad(int idx, int n, double kf) {
+    return idx == n / 2 ? 0.0 : ((double)idx - (double)(idx > n / 2) * n) * kf;
+}
+
+// k-space CIC on an ng x ng x (ng/2+1) table (Grid_FFT::get_cic_kspace,
+// grid_fft.hh:347-373: periodic in x and y, clamped at the last z plane)
+__device__ inline double cic_k(const double* __restrict__ T, int ng, double x, double y, double z) {
+    const int nc = ng / 2 + 1;
+    const int ix = (int)floor(x), iy = (int)floor(y), iz = (int)floor(z);
+    const double dx = x - ix, tx = 1.0 - dx, dy = y - iy, ty = 1.0 - dy, dz = z - iz, tz = 1.0 - dz;
+    const int ix1 = (ix + 1) % ng, iy1 = (iy + 1) % ng, iz1 = min(iz + 1, nc - 1);
+    auto at = [&](int a, int b, int c) { return T[((size_t)a * ng + b) * nc + c]; };
+    double v = 0.0;
+    v += at(ix, iy, iz) * tx * ty * tz;
+    v += at(ix, iy, iz1) * tx * ty * dz;
+    v += at(ix, iy1, iz) * tx * dy * tz;
+    v += at(ix, iy1, iz1) * tx * dy * dz;
+    v += at(ix1, iy, iz) * dx * ty * tz;
+    v += at(ix1, iy, iz1) * dx * ty * dz;
+    v += at(ix1, iy1, iz) * dx * dy * tz;
+    v += at(ix1, iy1, iz1) * dx * dy * dz;
+    return v;
+}
+
+// lattice_gradient::gradient for all three axes + vfac_corr
+// (particle_plt.hh:551-585), for N-grid mode (i, j, k) of a box of length L
+__device__ inline void plt_grad(const PltDev& P, int n, double L, int i, int j, int k, double g[3], double* vcorr) {
+    const int ng = P.ng;
+    const double mr = (double)ng / (double)n;
+    const double ix = i * mr, iy = j * mr, iz = k * mr;
+    const double kf = 2.0 * 3.14159265358979323846;
+    const double hn = (double)(ng / 2);
+    const double kx = (ix - (double)(ix > hn) * ng) * kf, ky = (iy - (double)(iy > hn) * ng) * kf,
+                 kz = (iz - (double)(iz > hn) * ng) * kf;
+    const double kr = sqrt(kx * kx + ky * ky + kz * kz);
+    const double kmod = kr / mr / L;
+    const double Dr = cic_k(P.Dr, ng, ix, iy, iz), Dt = cic_k(P.Dtheta, ng, ix, iy, iz),
+                 Dp = cic_k(P.Dphi, ng, ix, iy, iz);
+    const double kphi = kr > 0.0 ? atan2(ky, kx) : 0.0, kth = kr > 0.0 ? acos(kz / kr) : 0.0;
+    double st, ct, sp, cp;
+    sincos(kth, &st, &ct);
+    sincos(kphi, &sp, &cp);
+    g[0] = kmod * (Dr * st * cp + Dt * ct * cp - Dp * sp);
+    g[1] = kmod * (Dr * st * sp + Dt * ct * sp + Dp * cp);
+    g[2] = kmod * (Dr * ct - Dt * st);
+    if (vcorr) *vcorr = cic_k(P.vfac, ng, ix, iy, iz);
+}
+
+template <typename T>
+__global__ void k_assemble(cx<T>* __restrict__ out, AsmArgs a, double L, size_t ntot) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntot) return;
+    const int k = (int)(idx % a.nc);
+    const size_t r = idx / a.nc;
+    const int j = (int)(r % a.n1), i = (int)(r / a.n1);
+    const bool vel = a.comp >= 3;
+    const int d = a.comp % 3, dp = (d + 1) % 3, dpp = (d + 2) % 3;
+    double g[3], vc = 1.0;
+    if (a.plt.Dr) {
+        plt_grad(a.plt, a.n0, L, i, j, k, g, vel ? &vc : nullptr);
+    } else {
+        g[0] = fgrad(i, a.n0, a.kf0);
+        g[1] = fgrad(j, a.n1, a.kf1);
+        g[2] = fgrad(k, a.n2, a.kf2);
+    }
+    // potential sum in FP64 with a compensated (two-sum) accumulation
+    double sr = 0.0, si = 0.0, er = 0.0, ei = 0.0;
+    const double cf[3] = {a.c1, a.c2, a.c3};
+    for (int o = 0; o < a.order; ++o) {
+        const cx<T> v = static_cast<const cx<T>*>(a.phi[o])[idx];
+        const double xr = cf[o] * (double)v.x, xi = cf[o] * (double)v.y;
+        double t = sr + xr, bb = t - sr;
+        er += (sr - (t - bb)) + (xr - bb);
+        sr = t;
+        t = si + xi; bb = t - si;
+        ei += (si - (t - bb)) + (xi - bb);
+        si = t;
+    }
+    sr += er;
+    si += ei;
+    // (i g_d) * phitot
+    double vr = -g[d] * si, vi = g[d] * sr;
+    if (a.order > 2) {
+        const cx<T> a1 = static_cast<const cx<T>*>(a.A3[dpp])[idx], a2 = static_cast<const cx<T>*>(a.A3[dp])[idx];
+        // (i g_d') A3_d'' - (i g_d'') A3_d'
+        const double cr = (-g[dp] * (double)a1.y) - (-g[dpp] * (double)a2.y);
+        const double ci = (g[dp] * (double)a1.x) - (g[dpp] * (double)a2.x);
+        vr += a.cA * cr;
+        vi += a.cA * ci;
+    }
+    if (vel) { vr *= vc; vi *= vc; }
+    vr *= a.unit;
+    vi *= a.unit;
+    if (a.sh0 != 0.0 || a.sh1 != 0.0 || a.sh2 != 0.0) {
+        const double kx = ((double)i - (double)(i > a.n0 / 2) * a.n0) * a.kf0;
+        const double ky = ((double)j - (double)(j > a.n1 / 2) * a.n1) * a.kf1;
+        const double kz = ((double)k - (double)(k > a.n2 / 2) * a.n2) * a.kf2;
+        double s, c;
+        sincos(a.sh0 * kx + a.sh1 * ky + a.sh2 * kz, &s, &c);
+        const double tr = vr * c - vi * s, ti = vr * s + vi * c;
+        vr = tr;
+        vi = ti;
+    }
+    if (idx == 0) { vr = 0.0; vi = 0.0; }
+    out[idx] = cx<T>{(T)vr, (T)vi};
+}
+
+// particle value = q + field (positions, FP64) or field (velocities)
+template <typename T>
+__global__ void k_readout(const T* __restrict__ f, double* __restrict__ dst, int n, int idim, int is_pos, double lunit,
+                          double shift, size_t pbase) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t np = (size_t)n * n * n;
+    if (p >= np) return;
+    const int k = (int)(p % n);
+    const size_t r = p / n;
+    const int j = (int)(r % n), i = (int)(r / n);
+    double v = (double)f[r * (n + 2) + k];
+    if (is_pos) {
+        const int q = idim == 0 ? i : idim == 1 ? j : k;
+        v = (((double)q + shift) / (double)n) * lunit + v;
+    }
+    dst[3 * (pbase + p) + idim] = v;
+}
+
+// ------------------------------------------------------------- host -----
+
+void assemble_component(mfic_ctx ctx, int prec, const size_t n[3], double L, int order, int comp, const double cf[4],
+                        double unit, const PltDev& plt, const void* const phi[3], const void* const A3[3],
+                        void* out, const double shift_dx[3]) {
+    AsmArgs a{};
+    for (int o = 0; o < 3; ++o) { a.phi[o] = phi[o]; a.A3[o] = A3[o]; }
+    a.n0 = (int)n[0]; a.n1 = (int)n[1]; a.n2 = (int)n[2]; a.nc = (int)(n[2] / 2 + 1);
+    a.order = order;
+    a.comp = comp;
+    const double pi2 = 2.0 * 3.14159265358979323846;
+    a.kf0 = pi2 / L; a.kf1 = pi2 / L; a.kf2 = pi2 / L;
+    a.c1 = cf[0]; a.c2 = cf[1]; a.c3 = cf[2]; a.cA = cf[3];
+    a.unit = unit;
+    a.sh0 = shift_dx ? shift_dx[0] : 0.0;
+    a.sh1 = shift_dx ? shift_dx[1] : 0.0;
+    a.sh2 = shift_dx ? shift_dx[2] : 0.0;
+    a.plt = plt;
+    const size_t ntot = n[0] * n[1] * (n[2] / 2 + 1);
+    if (prec == MFIC_F64)
+        hipLaunchKernelGGL(k_assemble<double>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, (cx<double>*)out, a, L, ntot);
+    else
+        hipLaunchKernelGGL(k_assemble<float>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, (cx<float>*)out, a, L, ntot);
+    MFIC_LAUNCH_CHECK();
+}
+
+void lattice_readout_raw(mfic_ctx ctx, int prec, const void* field, int n, int idim, int is_pos, double lunit,
+                         double shift, size_t pbase, double* dst) {
+    const size_t np = (size_t)n * n * n;
+    if (prec == MFIC_F64)
+        hipLaunchKernelGGL(k_readout<double>, dim3(nblk(np, 256)), dim3(256), 0, ctx->stream, (const double*)field, dst, n,
+                           idim, is_pos, lunit, shift, pbase);
+    else
+        hipLaunchKernelGGL(k_readout<float>, dim3(nblk(np, 256)), dim3(256), 0, ctx->stream, (const float*)field, dst, n,
+                           idim, is_pos, lunit, shift, pbase);
+    MFIC_LAUNCH_CHECK();
+}
+
+}  // namespace mfic

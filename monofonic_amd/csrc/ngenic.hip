@@ -1,0 +1,315 @@
+// NGenIC white noise on the device (random_ngenic.cc:39-154).
+//
+// Each (i, j) column of the k-space grid is an independent RANLUX stream
+// reseeded from the seed table, so one thread generates one column.  The
+// RANLUX state (12 words of 48 bits) lives in registers; every 202-step cycle
+// is unrolled with static indices and the register names are rotated by 10
+// afterwards, so a cycle always starts at word 0 and the 12 outputs of the
+// cycle are words 0..11 in order.  The Gaussian transform evaluates log and
+// sin/cos in double-double and rounds once (same operation sequence as the
+// host seed-table code and the CPU oracle), giving bit-identical draws.
+//
+// This file is compiled with -ffp-contract=off: every fma is explicit.
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "fft_core.hpp"
+#include "mfic_internal.hpp"
+#include "ngenic.hpp"
+
+namespace mfic {
+
+// ----------------------------------------------------- double-double ----
+struct dd { double hi, lo; };
+
+__host__ __device__ inline dd two_sum(double a, double b) {
+    double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ inline dd fast_two_sum(double a, double b) {
+    double s = a + b;
+    return {s, b - (s - a)};
+}
+__host__ __device__ inline dd two_prod(double a, double b) {
+    double p = a * b;
+    return {p, fma(a, b, -p)};
+}
+__host__ __device__ inline dd dd_add(dd x, dd y) {
+    dd s = two_sum(x.hi, y.hi), t = two_sum(x.lo, y.lo);
+    double c = s.lo + t.hi;
+    dd v = fast_two_sum(s.hi, c);
+    c = t.lo + v.lo;
+    return fast_two_sum(v.hi, c);
+}
+__host__ __device__ inline dd dd_mul(dd x, dd y) {
+    dd p = two_prod(x.hi, y.hi);
+    double e = fma(x.hi, y.lo, x.lo * y.hi) + p.lo;
+    return fast_two_sum(p.hi, e);
+}
+__host__ __device__ inline dd dd_mul_d(dd x, double d) {
+    dd p = two_prod(x.hi, d);
+    double e = fma(x.lo, d, p.lo);
+    return fast_two_sum(p.hi, e);
+}
+__host__ __device__ inline dd dd_div(dd x, dd y) {
+    double q1 = x.hi / y.hi;
+    dd p = dd_mul_d(y, q1);
+    dd r = dd_add(x, dd{-p.hi, -p.lo});
+    double q2 = r.hi / y.hi;
+    p = dd_mul_d(y, q2);
+    r = dd_add(r, dd{-p.hi, -p.lo});
+    double q3 = r.hi / y.hi;
+    dd q = fast_two_sum(q1, q2);
+    return dd_add(q, dd{q3, 0.0});
+}
+__host__ __device__ inline dd dd_recip_d(double d) {
+    double hi = 1.0 / d;
+    double r = fma(-hi, d, 1.0);
+    return {hi, r / d};
+}
+
+// series coefficients, computed once on the host with the same operations as
+// the CPU oracle: log: 1/(2k+1), k = 0..26;  sin / cos Taylor terms
+struct CrTables {
+    double logc[27][2];
+    double sinc[16][2];
+    double cosc[16][2];
+};
+
+static CrTables make_tables() {
+    CrTables t;
+    for (int k = 0; k <= 26; ++k) {
+        dd r = dd_recip_d((double)(2 * k + 1));
+        t.logc[k][0] = r.hi; t.logc[k][1] = r.lo;
+    }
+    dd cs = {1.0, 0.0}, cc = {1.0, 0.0};
+    t.sinc[0][0] = 1.0; t.sinc[0][1] = 0.0; t.cosc[0][0] = 1.0; t.cosc[0][1] = 0.0;
+    for (int k = 1; k <= 15; ++k) {
+        cs = dd_div(cs, dd{(double)((2 * k) * (2 * k + 1)), 0.0});
+        cc = dd_div(cc, dd{(double)((2 * k - 1) * (2 * k)), 0.0});
+        t.sinc[k][0] = cs.hi; t.sinc[k][1] = cs.lo;
+        t.cosc[k][0] = cc.hi; t.cosc[k][1] = cc.lo;
+    }
+    return t;
+}
+
+__device__ inline double cr_log(double x, const CrTables& T) {
+    // x in (0, 1) here: positive, finite, normal
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
+    dd num = {m - 1.0, 0.0};
+    dd den = two_sum(m, 1.0);
+    dd s = dd_div(num, den);
+    dd t = dd_mul(s, s);
+    dd p = {T.logc[26][0], T.logc[26][1]};
+    for (int k = 25; k >= 0; --k) {
+        p = dd_mul(p, t);
+        p = dd_add(p, dd{T.logc[k][0], T.logc[k][1]});
+    }
+    dd lm = dd_mul(s, p);
+    lm.hi *= 2.0; lm.lo *= 2.0;
+    dd el = dd_mul_d(dd{6.93147180559945286227e-01, 2.31904681384629955842e-17}, (double)e);
+    dd r = dd_add(el, lm);
+    return r.hi + r.lo;
+}
+
+__device__ inline void cr_sincos(double x, double* sn, double* cs, const CrTables& T) {
+    const double PIO2_1 = 1.57079632679489655800e+00, PIO2_2 = 6.12323399573676603587e-17,
+                 PIO2_3 = -1.49738490485916983634e-33;
+    double qd = rint(x * 0.63661977236758134308);
+    dd r;
+    if (qd == 0.0) {
+        r = dd{x, 0.0};
+    } else {
+        dd p1 = two_prod(qd, PIO2_1), p2 = two_prod(qd, PIO2_2);
+        double p3 = qd * PIO2_3;
+        r = two_sum(x, -p1.hi);
+        r = dd_add(r, dd{-p1.lo, 0.0});
+        r = dd_add(r, dd{-p2.hi, -p2.lo});
+        r = dd_add(r, dd{-p3, 0.0});
+    }
+    dd t = dd_mul(r, r);
+    const int K = 15;
+    dd ps = {-T.sinc[K][0], -T.sinc[K][1]}, pc = {-T.cosc[K][0], -T.cosc[K][1]};  // K odd
+    for (int k = K - 1; k >= 0; --k) {
+        dd a = {T.sinc[k][0], T.sinc[k][1]}, b = {T.cosc[k][0], T.cosc[k][1]};
+        if (k & 1) { a.hi = -a.hi; a.lo = -a.lo; b.hi = -b.hi; b.lo = -b.lo; }
+        ps = dd_add(dd_mul(ps, t), a);
+        pc = dd_add(dd_mul(pc, t), b);
+    }
+    dd ss = dd_mul(ps, r);
+    double sv = ss.hi + ss.lo, cv = pc.hi + pc.lo;
+    long q = (long)qd;
+    switch (((q % 4) + 4) % 4) {
+    case 0: *sn = sv; *cs = cv; break;
+    case 1: *sn = cv; *cs = -sv; break;
+    case 2: *sn = -sv; *cs = -cv; break;
+    default: *sn = -cv; *cs = sv; break;
+    }
+}
+
+// ------------------------------------------------------------- RANLUX ----
+constexpr uint64_t kMask48 = (uint64_t(1) << 48) - 1;
+
+// gsl_rng_set seeding of ranlxd: 31-bit shift register (taps 0 and 18),
+// 48 complemented register bits per word.  Host form: the seed is held in a
+// C int exactly as there.
+inline void ranlux_seed_host(uint64_t x[12], unsigned long seed) {
+    if (seed == 0) seed = 1;
+    int i = (int)(uint32_t)(seed & 0xFFFFFFFFUL);
+    int xbit[31];
+    for (int k = 0; k < 31; ++k) { xbit[k] = i % 2; i /= 2; }
+    int ibit = 0, jbit = 18;
+    for (int k = 0; k < 12; ++k) {
+        uint64_t w = 0;
+        for (int l = 1; l <= 48; ++l) {
+            const int y = (xbit[ibit] + 1) % 2;
+            w = 2 * w + (uint64_t)(int64_t)y;
+            xbit[ibit] = (xbit[ibit] + xbit[jbit]) % 2;
+            ibit = (ibit + 1) % 31;
+            jbit = (jbit + 1) % 31;
+        }
+        x[k] = w & kMask48;
+    }
+}
+
+// Device form for seeds in [0, 2^31): the register is a 31-bit mask so that
+// nothing is indexed dynamically (the state stays in VGPRs).
+__device__ inline void ranlux_seed_dev(uint64_t x[12], uint32_t seed) {
+    uint32_t bits = (seed == 0 ? 1u : seed) & 0x7fffffffu;
+    int ibit = 0, jbit = 18;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        uint64_t w = 0;
+        for (int l = 0; l < 48; ++l) {
+            const uint32_t bi = (bits >> ibit) & 1u, bj = (bits >> jbit) & 1u;
+            w = 2 * w + (uint64_t)(1u - bi);
+            bits = (bits & ~(1u << ibit)) | ((bi ^ bj) << ibit);
+            ibit = ibit == 30 ? 0 : ibit + 1;
+            jbit = jbit == 30 ? 0 : jbit + 1;
+        }
+        x[k] = w;
+    }
+}
+
+// x_ir <- x_jr - x_ir - c  (mod 2^48), jr = ir + 7 (mod 12)
+#define MFIC_SWB(IR)                                                          \
+    {                                                                         \
+        uint64_t d_ = x[((IR) + 7) % 12] + (uint64_t(1) << 48) - x[(IR)] - c; \
+        c = 1 - (int)(d_ >> 48);                                              \
+        x[(IR)] = d_ & kMask48;                                               \
+    }
+
+// one luxury-202 cycle starting at word 0; afterwards the words are renamed
+// so that the 12 outputs (the last 12 produced, oldest first) are x[0..11]
+// and the next cycle again starts at word 0
+__host__ __device__ inline void ranlux_cycle202(uint64_t x[12], int& c) {
+    for (int blk = 0; blk < 16; ++blk) {
+#pragma unroll
+        for (int r = 0; r < 12; ++r) MFIC_SWB(r)
+    }
+#pragma unroll
+    for (int r = 0; r < 10; ++r) MFIC_SWB(r)
+    const uint64_t a = x[10], b = x[11];
+#pragma unroll
+    for (int r = 11; r >= 2; --r) x[r] = x[r - 2];
+    x[0] = a;
+    x[1] = b;
+}
+#undef MFIC_SWB
+
+__device__ inline uint64_t sel12(const uint64_t x[12], int c) {
+    uint64_t v = x[0];
+#pragma unroll
+    for (int r = 1; r < 12; ++r) v = (c == r) ? x[r] : v;
+    return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, CrTables tabs) {
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long)n * n) return;
+    const int i = (int)(col / n), j = (int)(col % n);
+    const int nc = n / 2 + 1, nh = n / 2;
+    const int ii = i > 0 ? n - i : 0, jj = j > 0 ? n - j : 0;
+    uint64_t x[12];
+    ranlux_seed_dev(x, table[(long)i * n + j]);
+    int carry = 0, pos = 12;  // pos == 12: the next draw runs a cycle
+    for (int k = 0; k < nc; ++k) {
+        if (pos == 12) { ranlux_cycle202(x, carry); pos = 0; }
+        const double phase = (double)sel12(x, pos++) * (1.0 / 281474976710656.0) * 2 * 3.14159265358979323846;
+        double ampl;
+        do {
+            if (pos == 12) { ranlux_cycle202(x, carry); pos = 0; }
+            ampl = (double)sel12(x, pos++) * (1.0 / 281474976710656.0);
+        } while (ampl == 0 || ampl == 1);
+        if (i == nh || j == nh || k == nh) continue;
+        if (i == 0 && j == 0 && k == 0) continue;
+        ampl = sqrt(-cr_log(ampl, tabs));
+        double sn, cs;
+        cr_sincos(phase, &sn, &cs, tabs);
+        const double zr = ampl * cs, zi = ampl * sn;
+        if (k > 0) {
+            g[((long)i * n + j) * nc + k] = cx<T>{(T)zr, (T)zi};
+        } else if (i == 0) {
+            if (j < nh) {
+                g[((long)i * n + j) * nc] = cx<T>{(T)zr, (T)zi};
+                g[((long)i * n + jj) * nc] = cx<T>{(T)zr, (T)-zi};
+            }
+        } else if (i < nh) {
+            g[((long)i * n + j) * nc] = cx<T>{(T)zr, (T)zi};
+            g[((long)ii * n + jj) * nc] = cx<T>{(T)zr, (T)-zi};
+        }
+    }
+}
+
+// ------------------------------------------------------------- host -----
+
+namespace {
+struct HostRanlux {
+    uint64_t x[12];
+    int carry = 0, pos = 12;
+    explicit HostRanlux(unsigned long seed) { ranlux_seed_host(x, seed); }
+    double uniform() {
+        if (pos == 12) { ranlux_cycle202(x, carry); pos = 0; }
+        return (double)x[pos++] * (1.0 / 281474976710656.0);
+    }
+};
+}  // namespace
+
+void ngenic_seedtable(long seed, size_t n, uint32_t* T) {
+    HostRanlux r((unsigned long)seed);
+    auto draw = [&r]() { return (uint32_t)(2147483647.0 * r.uniform()); };
+    for (size_t q = 0; q < n * n; ++q) T[q] = 0;
+    for (size_t i = 0; i < n / 2; ++i) {
+        for (size_t j = 0; j < i; j++) T[i * n + j] = draw();
+        for (size_t j = 0; j < i + 1; j++) T[j * n + i] = draw();
+        for (size_t j = 0; j < i; j++) T[(n - 1 - i) * n + j] = draw();
+        for (size_t j = 0; j < i + 1; j++) T[(n - 1 - j) * n + i] = draw();
+        for (size_t j = 0; j < i; j++) T[i * n + (n - 1 - j)] = draw();
+        for (size_t j = 0; j < i + 1; j++) T[j * n + (n - 1 - i)] = draw();
+        for (size_t j = 0; j < i; j++) T[(n - 1 - i) * n + (n - 1 - j)] = draw();
+        for (size_t j = 0; j < i + 1; j++) T[(n - 1 - j) * n + (n - 1 - i)] = draw();
+    }
+}
+
+void ngenic_fill(mfic_grid g, const uint32_t* dev_table) {
+    MFIC_CHECK(g->n[0] == g->n[1] && g->n[1] == g->n[2], MFIC_ERR_SIZE, "NGenIC needs a cubic grid");
+    MFIC_CHECK(g->n[0] % 2 == 0, MFIC_ERR_SIZE, "NGenIC needs an even grid size");
+    static const CrTables tabs = make_tables();
+    const int n = (int)g->n[0];
+    const long cols = (long)n * n;
+    const int bs = 64;
+    if (g->prec == MFIC_F64)
+        hipLaunchKernelGGL(k_ngenic_fill<double>, dim3((unsigned)((cols + bs - 1) / bs)), dim3(bs), 0, g->ctx->stream,
+                           (cx<double>*)g->d, dev_table, n, tabs);
+    else
+        hipLaunchKernelGGL(k_ngenic_fill<float>, dim3((unsigned)((cols + bs - 1) / bs)), dim3(bs), 0, g->ctx->stream,
+                           (cx<float>*)g->d, dev_table, n, tabs);
+    MFIC_LAUNCH_CHECK();
+}
+
+}  // namespace mfic

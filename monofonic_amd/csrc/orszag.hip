@@ -1,0 +1,346 @@
+// Orszag 3/2-rule dealiased products (convolution.hh:412-651) and the LPT
+// source block of ic_generator::run (ic_generator.cc:523-629).
+//
+// Two entry levels:
+//  * convolve2 / convolve3: the reference's per-product schedule (pad both
+//    operands, two c2r, product, r2c, unpad + output operator), kept for the
+//    drop-in OrszagConvolver interface;
+//  * lpt_sources: the reorganised schedule used by the pipeline.  Each
+//    Hessian of phi1 and phi2 is transformed to padded real space once, all
+//    products sharing one truncation are summed (in FP64) before a single
+//    r2c, and the cubic 3a terms keep their two-stage truncation
+//    trunc(trunc(a b) c) (convolution.hh:525-532) with the inner products
+//    grouped by their outer factor.  Equal to the reference schedule in exact
+//    arithmetic (every step is linear in the grouped quantity).
+#include <cmath>
+
+#include "fft_core.hpp"
+#include "kspace.hpp"
+#include "mfic_internal.hpp"
+#include "orszag.hpp"
+
+namespace mfic {
+
+static inline unsigned nblk(size_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+struct Dims {
+    int n0, n1, n2, nc;       // N-grid
+    int p0, p1, p2, pc;       // padded grid
+    double kf0, kf1, kf2;     // 2 pi / L
+};
+
+// Grid_FFT::gradient imaginary part (grid_fft.hh:376-385): 0 at index n/2
+__device__ __forceinline__ double grad_comp(int idx, int n, double kf) {
+    return idx == n / 2 ? 0.0 : ((double)idx - (double)(idx > n / 2) * n) * kf;
+}
+
+template <typename T> struct DevOperand {
+    const cx<T>* f;
+    int kind, d0, d1, d2, d3;
+};
+
+// operand value at N-grid mode (i,j,k) (convolution.hh:74-275)
+template <typename T>
+__device__ __forceinline__ void operand_val(const DevOperand<T>& o, const Dims& D, int i, int j, int k, size_t idx,
+                                            double& re, double& im) {
+    const cx<T> v = o.f[idx];
+    const double fr = (double)v.x, fi = (double)v.y;
+    double g[3];
+    g[0] = grad_comp(i, D.n0, D.kf0);
+    g[1] = grad_comp(j, D.n1, D.kf1);
+    g[2] = grad_comp(k, D.n2, D.kf2);
+    double s;
+    switch (o.kind) {
+    case MFIC_OPND_HESS: s = -(g[o.d0] * g[o.d1]); break;
+    case MFIC_OPND_SUM_HESS: s = -(g[o.d0] * g[o.d1]) + -(g[o.d2] * g[o.d3]); break;
+    case MFIC_OPND_DIFF_HESS: s = -(g[o.d0] * g[o.d1]) - -(g[o.d2] * g[o.d3]); break;
+    case MFIC_OPND_GRAD: re = -g[o.d0] * fi; im = g[o.d0] * fr; return;
+    default: re = fr; im = fi; return;
+    }
+    re = s * fr;
+    im = s * fi;
+}
+
+// padded index -> source index, -1 inside the zero band (pad_insert,
+// convolution.hh:562-571: i > N/2 goes to i + N/2)
+__device__ __forceinline__ int pad_src(int ip, int n) {
+    if (ip <= n / 2) return ip;
+    if (ip > n) return ip - n / 2;
+    return -1;
+}
+
+// write the whole padded k-space array: operand * (3/2)^{3/2} or 0
+template <typename T>
+__global__ void k_pad_insert(cx<T>* __restrict__ fp, DevOperand<T> o, Dims D, double rfac, size_t ntot) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntot) return;
+    const int kp = (int)(idx % D.pc);
+    const size_t r = idx / D.pc;
+    const int jp = (int)(r % D.p1), ip = (int)(r / D.p1);
+    const int i = pad_src(ip, D.n0), j = pad_src(jp, D.n1);
+    cx<T> out = {0, 0};
+    if (i >= 0 && j >= 0 && kp <= D.n2 / 2) {
+        double re, im;
+        operand_val(o, D, i, j, kp, ((size_t)i * D.n1 + j) * D.nc + kp, re, im);
+        out = cx<T>{(T)(re * rfac), (T)(im * rfac)};
+    }
+    fp[idx] = out;
+}
+
+// unpad + Nyquist zeroing + output operator (convolution.hh:600-636,
+// operators.hh:25-35)
+template <typename T>
+__global__ void k_unpad(const cx<T>* __restrict__ fp, cx<T>* __restrict__ out, Dims D, double inv_rfac, int op,
+                        double coef, size_t ntot) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntot) return;
+    const int k = (int)(idx % D.nc);
+    const size_t r = idx / D.nc;
+    const int j = (int)(r % D.n1), i = (int)(r / D.n1);
+    const int ip = i > D.n0 / 2 ? i + D.n0 / 2 : i, jp = j > D.n1 / 2 ? j + D.n1 / 2 : j;
+    double re = 0.0, im = 0.0;
+    if (!(i == D.n0 / 2 || j == D.n1 / 2 || k == D.n2 / 2)) {
+        const cx<T> v = fp[((size_t)ip * D.p1 + jp) * D.pc + k];
+        re = (double)v.x * inv_rfac;
+        im = (double)v.y * inv_rfac;
+    }
+    cx<T> o = out[idx];
+    switch (op) {
+    case MFIC_OP_ASSIGN: o = cx<T>{(T)re, (T)im}; break;
+    case MFIC_OP_ADD: o = cx<T>{(T)((double)o.x + re), (T)((double)o.y + im)}; break;
+    case MFIC_OP_SUB: o = cx<T>{(T)((double)o.x - re), (T)((double)o.y - im)}; break;
+    default: o = cx<T>{(T)((double)o.x + re * coef), (T)((double)o.y + im * coef)}; break;
+    }
+    out[idx] = o;
+}
+
+template <typename T> __global__ void k_mul(T* __restrict__ a, const T* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] *= b[i];
+}
+
+// out = sum_t c_t A_t B_t over real padded elements, accumulated in FP64
+struct ProdTerms {
+    const void* a[16];
+    const void* b[16];
+    double c[16];
+    int n;
+};
+template <typename T>
+__global__ void k_prodsum(T* __restrict__ out, ProdTerms P, size_t ntot) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    double acc = 0.0;
+    for (int t = 0; t < P.n; ++t)
+        acc += P.c[t] * ((double)static_cast<const T*>(P.a[t])[i] * (double)static_cast<const T*>(P.b[t])[i]);
+    out[i] = (T)acc;
+}
+
+// ------------------------------------------------------------- host -----
+
+static Dims dims_of(const size_t n[3], const double L[3]) {
+    Dims D;
+    D.n0 = (int)n[0]; D.n1 = (int)n[1]; D.n2 = (int)n[2]; D.nc = D.n2 / 2 + 1;
+    D.p0 = 3 * D.n0 / 2; D.p1 = 3 * D.n1 / 2; D.p2 = 3 * D.n2 / 2; D.pc = D.p2 / 2 + 1;
+    const double pi2 = 2.0 * 3.14159265358979323846;
+    D.kf0 = pi2 / L[0]; D.kf1 = pi2 / L[1]; D.kf2 = pi2 / L[2];
+    return D;
+}
+
+ConvCore::ConvCore(mfic_ctx c, const size_t n[3], const double L[3], int p) : ctx(c), prec(p) {
+    for (int i = 0; i < 3; ++i) {
+        MFIC_CHECK(n[i] % 2 == 0, MFIC_ERR_SIZE, "Orszag convolver for LPTorder>1 requires even grid size!");
+        N[i] = n[i];
+        Np[i] = 3 * n[i] / 2;
+        len[i] = L[i];
+    }
+    MFIC_CHECK(Np[2] % 2 == 0, MFIC_ERR_SIZE, "padded contiguous axis 3N/2 must be even (N multiple of 4)");
+    for (int i = 0; i < 3; ++i)
+        MFIC_CHECK(fft_length_supported(N[i]) && fft_length_supported(Np[i]), MFIC_ERR_SIZE,
+                   "unsupported grid size for the Orszag convolver");
+}
+
+size_t ConvCore::padded_reals() const { return Np[0] * Np[1] * (Np[2] + 2); }
+size_t ConvCore::grid_reals() const { return N[0] * N[1] * (N[2] + 2); }
+size_t ConvCore::esize() const { return prec == MFIC_F64 ? 8 : 4; }
+
+template <typename T>
+static void pad_insert_t(const ConvCore& c, const OperandDesc& o, void* fp) {
+    Dims D = dims_of(c.N, c.len);
+    DevOperand<T> od{static_cast<const cx<T>*>(o.f), o.kind, o.d[0], o.d[1], o.d[2], o.d[3]};
+    const size_t ntot = (size_t)D.p0 * D.p1 * D.pc;
+    const double rfac = std::pow(1.5, 1.5);
+    hipLaunchKernelGGL(k_pad_insert<T>, dim3(nblk(ntot, 256)), dim3(256), 0, c.ctx->stream, (cx<T>*)fp, od, D, rfac, ntot);
+    MFIC_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void unpad_t(const ConvCore& c, const void* fp, void* out, int op, double coef) {
+    Dims D = dims_of(c.N, c.len);
+    const size_t ntot = (size_t)D.n0 * D.n1 * D.nc;
+    const double rfac = std::sqrt((double)(c.Np[0] * c.Np[1] * c.Np[2])) / std::sqrt((double)(c.N[0] * c.N[1] * c.N[2]));
+    hipLaunchKernelGGL(k_unpad<T>, dim3(nblk(ntot, 256)), dim3(256), 0, c.ctx->stream, (const cx<T>*)fp, (cx<T>*)out, D,
+                       1.0 / rfac, op, coef, ntot);
+    MFIC_LAUNCH_CHECK();
+}
+
+void ConvCore::pad_insert(const OperandDesc& o, void* fp) const {
+    if (prec == MFIC_F64) pad_insert_t<double>(*this, o, fp);
+    else pad_insert_t<float>(*this, o, fp);
+}
+void ConvCore::unpad(const void* fp, void* out, int op, double coef) const {
+    if (prec == MFIC_F64) unpad_t<double>(*this, fp, out, op, coef);
+    else unpad_t<float>(*this, fp, out, op, coef);
+}
+void ConvCore::c2r_padded(void* fp) const {
+    const double s = 1.0 / std::sqrt((double)Np[0] * (double)Np[1] * (double)Np[2]);
+    if (prec == MFIC_F64) fft_c2r<double>(ctx, (double*)fp, Np, s);
+    else fft_c2r<float>(ctx, (float*)fp, Np, s);
+}
+void ConvCore::r2c_padded(void* fp) const {
+    const double s = 1.0 / std::sqrt((double)Np[0] * (double)Np[1] * (double)Np[2]);
+    if (prec == MFIC_F64) fft_r2c<double>(ctx, (double*)fp, Np, s);
+    else fft_r2c<float>(ctx, (float*)fp, Np, s);
+}
+void ConvCore::mul(void* a, const void* b) const {
+    const size_t n = padded_reals();
+    if (prec == MFIC_F64)
+        hipLaunchKernelGGL(k_mul<double>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (double*)a, (const double*)b, n);
+    else
+        hipLaunchKernelGGL(k_mul<float>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (float*)a, (const float*)b, n);
+    MFIC_LAUNCH_CHECK();
+}
+void ConvCore::prodsum(void* out, const std::vector<Term>& terms) const {
+    MFIC_CHECK(terms.size() <= 16, MFIC_ERR_ARG, "too many product terms");
+    ProdTerms P{};
+    P.n = (int)terms.size();
+    for (size_t t = 0; t < terms.size(); ++t) { P.a[t] = terms[t].a; P.b[t] = terms[t].b; P.c[t] = terms[t].c; }
+    const size_t n = padded_reals();
+    if (prec == MFIC_F64)
+        hipLaunchKernelGGL(k_prodsum<double>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (double*)out, P, n);
+    else
+        hipLaunchKernelGGL(k_prodsum<float>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (float*)out, P, n);
+    MFIC_LAUNCH_CHECK();
+}
+
+// convolve2 in the reference's schedule
+void ConvCore::convolve2(const OperandDesc& a, const OperandDesc& b, void* out, int op, double coef, void* f1p,
+                         void* f2p) const {
+    pad_insert(a, f1p);
+    pad_insert(b, f2p);
+    c2r_padded(f1p);
+    c2r_padded(f2p);
+    mul(f2p, f1p);
+    r2c_padded(f2p);
+    unpad(f2p, out, op, coef);
+}
+
+void ConvCore::convolve3(const OperandDesc& a, const OperandDesc& b, const OperandDesc& c, void* out, int op,
+                         double coef, void* f1p, void* f2p, void* fbuf) const {
+    convolve2(a, b, fbuf, MFIC_OP_ASSIGN, 0.0, f1p, f2p);
+    OperandDesc id{fbuf, MFIC_OPND_IDENT, {0, 0, 0, 0}};
+    convolve2(id, c, out, op, coef, f1p, f2p);
+}
+
+// ------------------------------------------------- reorganised sources ----
+
+void lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_grid phi3, mfic_grid A3x,
+                 mfic_grid A3y, mfic_grid A3z) {
+    if (order < 2) return;
+    ConvCore C(ctx, phi1->n, phi1->L, phi1->prec);
+    const size_t pb = C.padded_reals() * C.esize();
+    // xx yy zz xy xz yz
+    static const int HD[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    auto hidx = [](int a, int b) {
+        if (a > b) std::swap(a, b);
+        if (a == b) return a;
+        return a == 0 ? (b == 1 ? 3 : 4) : 5;
+    };
+    void* H1[6];
+    for (int h = 0; h < 6; ++h) H1[h] = ctx_scratch(ctx, "H1_" + std::to_string(h), pb);
+    void* work = ctx_scratch(ctx, "pad_work", pb);
+    for (int h = 0; h < 6; ++h) {
+        C.pad_insert(OperandDesc{phi1->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, H1[h]);
+        C.c2r_padded(H1[h]);
+    }
+    // phi2 source (ic_generator.cc:536-541)
+    {
+        using T = ConvCore::Term;
+        std::vector<T> t = {{H1[0], H1[1], 1.0}, {H1[0], H1[2], 1.0}, {H1[1], H1[2], 1.0},
+                            {H1[3], H1[3], -1.0}, {H1[4], H1[4], -1.0}, {H1[5], H1[5], -1.0}};
+        C.prodsum(work, t);
+        C.r2c_padded(work);
+        C.unpad(work, phi2->d, MFIC_OP_ASSIGN, 0.0);
+        phi2->space = MFIC_KSPACE;
+        grid_inverse_laplacian(phi2);
+    }
+    if (order < 3) return;
+    void* H2[6];
+    for (int h = 0; h < 6; ++h) H2[h] = ctx_scratch(ctx, "H2_" + std::to_string(h), pb);
+    for (int h = 0; h < 6; ++h) {
+        C.pad_insert(OperandDesc{phi2->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, H2[h]);
+        C.c2r_padded(H2[h]);
+    }
+    // 3a inner products, truncated once and brought back to padded real space:
+    // Qzz = xx yy - xy^2 (outer zz), Qyz = xy xz (outer yz, x2), Qxx = yz^2
+    // (outer xx, -1), Qyy = xz^2 (outer yy, -1)   (ic_generator.cc:581-585)
+    using Tm = ConvCore::Term;
+    const std::vector<Tm> inner[4] = {{{H1[0], H1[1], 1.0}, {H1[3], H1[3], -1.0}},
+                                      {{H1[3], H1[4], 1.0}},
+                                      {{H1[5], H1[5], 1.0}},
+                                      {{H1[4], H1[4], 1.0}}};
+    void* Q[4];
+    const size_t gb = C.grid_reals() * C.esize();
+    void* qk = ctx_scratch(ctx, "qk", gb);
+    for (int q = 0; q < 4; ++q) {
+        Q[q] = ctx_scratch(ctx, "Q_" + std::to_string(q), pb);
+        C.prodsum(Q[q], inner[q]);
+        C.r2c_padded(Q[q]);
+        C.unpad(Q[q], qk, MFIC_OP_ASSIGN, 0.0);
+        C.pad_insert(OperandDesc{qk, MFIC_OPND_IDENT, {0, 0, 0, 0}}, Q[q]);
+        C.c2r_padded(Q[q]);
+    }
+    // phi3 source: 3a outer products + 3b (ic_generator.cc:581-609)
+    {
+        const double a = -5.0 / 7.0, b = 10.0 / 7.0;
+        std::vector<Tm> t = {{Q[0], H1[2], 1.0}, {Q[1], H1[5], 2.0}, {Q[2], H1[0], -1.0}, {Q[3], H1[1], -1.0},
+                             {H1[0], H2[1], a}, {H1[0], H2[2], a}, {H1[1], H2[2], a}, {H1[1], H2[0], a},
+                             {H1[2], H2[0], a}, {H1[2], H2[1], a},
+                             {H1[3], H2[3], b}, {H1[4], H2[4], b}, {H1[5], H2[5], b}};
+        C.prodsum(work, t);
+        C.r2c_padded(work);
+        C.unpad(work, phi3->d, MFIC_OP_ASSIGN, 0.0);
+        phi3->space = MFIC_KSPACE;
+        grid_inverse_laplacian(phi3);
+    }
+    // A3 (ic_generator.cc:614-627)
+    mfic_grid A3[3] = {A3x, A3y, A3z};
+    for (int d = 0; d < 3; ++d) {
+        const int dp = (d + 1) % 3, dpp = (d + 2) % 3;
+        std::vector<Tm> t = {{H2[hidx(d, dp)], H1[hidx(d, dpp)], 1.0},
+                             {H2[hidx(d, dpp)], H1[hidx(d, dp)], -1.0},
+                             {H1[hidx(dp, dpp)], H2[hidx(dp, dp)], 1.0},
+                             {H1[hidx(dp, dpp)], H2[hidx(dpp, dpp)], -1.0},
+                             {H2[hidx(dp, dpp)], H1[hidx(dp, dp)], -1.0},
+                             {H2[hidx(dp, dpp)], H1[hidx(dpp, dpp)], 1.0}};
+        C.prodsum(work, t);
+        C.r2c_padded(work);
+        C.unpad(work, A3[d]->d, MFIC_OP_ASSIGN, 0.0);
+        A3[d]->space = MFIC_KSPACE;
+        grid_inverse_laplacian(A3[d]);
+    }
+}
+
+template <typename T> __global__ void k_add(T* __restrict__ a, const T* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] += b[i];
+}
+void add_inplace(mfic_ctx ctx, int prec, void* a, const void* b, size_t n) {
+    if (prec == MFIC_F64)
+        hipLaunchKernelGGL(k_add<double>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (double*)a, (const double*)b, n);
+    else
+        hipLaunchKernelGGL(k_add<float>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (float*)a, (const float*)b, n);
+    MFIC_LAUNCH_CHECK();
+}
+
+}  // namespace mfic

@@ -1,0 +1,143 @@
+"""Device LPT path vs the CPU FP64 oracle on the same seeded white noise.
+
+FP64 device results must agree with the oracle to rel-L2 1e-10 (different FFT
+algorithms and summation orders); FP32 device results to rel-L2 2e-5 and
+max-abs relative error 5e-4 (tolerances for the FP32 bulk path).
+"""
+import numpy as np
+import pytest
+
+from monofonic_amd import F32, F64, Grid, _lib as L, lpt
+from oracle import oracle as O
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+TOL = {F64: 1e-10, F32: 2e-5}
+
+
+def _k_grid(ctx, arr, prec, box):
+    n = arr.shape[0]
+    g = Grid(ctx, (n,) * 3, (box,) * 3, prec=prec)
+    g.upload(arr if prec == F64 else arr.astype(np.float32))
+    g.set_space(L.KSPACE)
+    return g
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_phi1_matches_oracle(ctx, prec):
+    n, box, seed = 32, 100.0, 12345
+    cfg = O.ic_cfg(n, box, 1, seed)
+    oc = O.baseline_cosmology(49.0)
+    wn = O.ngenic_fill(seed, n)
+    phi, _, _, _ = O.potentials(cfg, oc, wn)
+    coef = O.lpt_coefs(cfg, oc)
+    g = Grid(ctx, (n,) * 3, (box,) * 3, prec=prec)
+    lpt.ngenic_fill(g, seed)
+    lpt.phi1_from_noise(g, g, lpt.baseline_cosmology(49.0))
+    got = g.download_k() * coef.g1
+    assert rel_l2(got, O.kview(phi)) < TOL[prec]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("kinds", [("H", (0, 0), "H", (1, 1)), ("H", (0, 1), "S", (1, 1, 2, 2)),
+                                   ("D", (1, 2, 0, 0), "H", (2, 2)), ("G", (0,), "G", (2,))])
+def test_convolve2_matches_oracle(ctx, prec, kinds):
+    n, box = 16, 50.0
+    rng = np.random.default_rng(3)
+    a = O.ngenic_fill(11, n)
+    b = O.ngenic_fill(12, n)
+    kmap = {"H": (L.HESS, O.HESS), "S": (L.SUM_HESS, O.SUM_HESS), "D": (L.DIFF_HESS, O.DIFF_HESS),
+            "G": (L.GRAD, O.GRAD)}
+    out_ref = O.empty_field(n)
+    O.kview(out_ref)[:] = rng.standard_normal((n, n, n // 2 + 1))
+    init = out_ref.copy()
+    O.convolve2(n, box, O.operand(a, kmap[kinds[0]][1], kinds[1]), O.operand(b, kmap[kinds[2]][1], kinds[3]),
+                out_ref, O.MADD, 0.75)
+    ga, gb = _k_grid(ctx, a, prec, box), _k_grid(ctx, b, prec, box)
+    go = _k_grid(ctx, init, prec, box)
+    conv = lpt.OrszagConvolver(ctx, (n,) * 3, (box,) * 3, prec)
+    conv.convolve2(lpt._operand(ga, kmap[kinds[0]][0], kinds[1]), lpt._operand(gb, kmap[kinds[2]][0], kinds[3]),
+                   go, (L.OP_MADD, 0.75))
+    got = go.download_k()
+    assert rel_l2(got - O.kview(init), O.kview(out_ref) - O.kview(init)) < 10 * TOL[prec]
+    conv.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_convolve3_matches_oracle(ctx, prec):
+    n, box = 16, 50.0
+    a, b, c = (O.ngenic_fill(s, n) for s in (21, 22, 23))
+    out_ref = O.empty_field(n)
+    O.convolve3(n, box, O.operand(a, O.HESS, (0, 1)), O.operand(b, O.HESS, (1, 1)), O.operand(c, O.HESS, (2, 0)),
+                out_ref, O.ASSIGN)
+    g = [_k_grid(ctx, x, prec, box) for x in (a, b, c)]
+    go = Grid(ctx, (n,) * 3, (box,) * 3, prec=prec, space=L.KSPACE)
+    conv = lpt.OrszagConvolver(ctx, (n,) * 3, (box,) * 3, prec)
+    conv.convolve3(lpt._operand(g[0], L.HESS, (0, 1)), lpt._operand(g[1], L.HESS, (1, 1)),
+                   lpt._operand(g[2], L.HESS, (2, 0)), go, L.OP_ASSIGN)
+    assert rel_l2(go.download_k(), O.kview(out_ref)) < 10 * TOL[prec]
+    conv.close()
+
+
+def _oracle_run(n, box, z, order, seed, lattice=0, plt=0, particles=False):
+    cfg = O.ic_cfg(n, box, order, seed, lattice=lattice, plt=plt)
+    oc = O.baseline_cosmology(z)
+    return O.ic_run(cfg, oc, particles=particles), cfg, oc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_potentials_match_oracle(ctx, order, prec):
+    n, box, z, seed = 32, 100.0, 24.0, 12345
+    cfg = O.ic_cfg(n, box, order, seed)
+    oc = O.baseline_cosmology(z)
+    pots = O.potentials(cfg, oc, O.ngenic_fill(seed, n))
+    coef = O.lpt_coefs(cfg, oc)
+    run = lpt.ICRun(ctx, lpt.ICConfig(n=n, box=box, zstart=z, lpt_order=order, prec=prec, seed=seed))
+    run.execute()
+    scale = [coef.g1, coef.g2, coef.g3, coef.g3c, coef.g3c, coef.g3c]
+    refs = [pots[0], pots[1], pots[2]] + list(pots[3])
+    for w in range(1 if order == 1 else 2 if order == 2 else 6):
+        got = run.potential(w).download_k() * scale[w]
+        err = rel_l2(got, O.kview(refs[w]))
+        print(f"potential {w} order {order} prec {prec}: rel-L2 {err:.3e}")
+        assert err < (50 if w > 0 else 1) * TOL[prec], (w, err)
+    run.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_fields_match_oracle(ctx, order, prec):
+    n, box, z, seed = 32, 100.0, 24.0, 12345
+    (fields, _, _, _), cfg, oc = _oracle_run(n, box, z, order, seed)
+    run = lpt.ICRun(ctx, lpt.ICConfig(n=n, box=box, zstart=z, lpt_order=order, prec=prec, seed=seed))
+    run.execute()
+    for c in range(6):
+        got = run.field(c).download_r()
+        ref = O.rview(fields[c])
+        err = rel_l2(got, ref)
+        print(f"field {c} order {order} prec {prec}: rel-L2 {err:.3e}")
+        assert err < 50 * TOL[prec], (c, err)
+    run.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_particles_match_oracle(ctx, lattice):
+    n, box, z, seed = 16, 100.0, 24.0, 777
+    (_, pos, vel, _), cfg, oc = _oracle_run(n, box, z, 2, seed, lattice=lattice, particles=True)
+    run = lpt.ICRun(ctx, lpt.ICConfig(n=n, box=box, zstart=z, lpt_order=2, prec=F64, seed=seed, particles=True,
+                                      lattice=lattice))
+    run.execute()
+    p, v = run.particles()
+    assert np.abs(p - pos.reshape(-1, 3)).max() < 1e-10
+    assert rel_l2(v, vel.reshape(-1, 3)) < 1e-9
+    run.close()
