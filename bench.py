@@ -1,0 +1,197 @@
+"""Benchmark: 3LPT + PLT initial conditions, bcc lattice, 512^3 grid (768^3
+padded), box 1000 Mpc/h, z_start = 24, FP32 bulk with FP64 q + Psi, on one
+MI355X.  One "step" is the whole hot path for one box: NGenIC white noise,
+phi1, the 2LPT/3LPT sources (Orszag 3/2 products) and inverse Laplacians,
+the six displacement / velocity components with PLT corrections, their
+c2r transforms and the bcc particle readout (2 x 512^3 particles).  Plan-time
+work (seed table, PLT tables, cosmology scalars) runs once before timing.
+
+Prints one JSON line.  Extra objects: "roofline" (dominant kernel, HIP-event
+timing on the library stream during the timed steps), "cpu_baseline" (the
+CPU FP64 oracle on a bounded 128^3 sample of the same configuration),
+"fp64" (same workload with FP64 bulk) and "accuracy" (FP32 vs FP64 device
+fields, rel-L2 / max error per component).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+METRIC = "3LPT IC-gen cells/s at 512^3, FP32 vs FP64 on 1 GPU; FFT HBM GB/s % of peak"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def device_sync():
+    from monofonic_amd.lpt import _hip
+    hip = _hip()
+    hip.hipDeviceSynchronize.restype = C.c_int
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
+def prof_report(ctx):
+    from monofonic_amd import _lib as L
+    buf = C.create_string_buffer(1 << 20)
+    L.call("mfic_prof_report", ctx.h, buf, len(buf))
+    return json.loads(buf.value.decode())
+
+
+def time_steps(ctx, run, steps, warmup, profile):
+    from monofonic_amd import _lib as L
+    for _ in range(warmup):
+        run.execute(blocking=True)
+    device_sync()
+    L.call("mfic_prof_enable", ctx.h, 1 if profile else 0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.execute(blocking=False)
+    ctx.sync()
+    device_sync()
+    t1 = time.perf_counter()
+    L.call("mfic_prof_enable", ctx.h, 0)
+    rep = prof_report(ctx) if profile else {}
+    return (t1 - t0) / steps * 1e3, rep
+
+
+def roofline(rep, steps):
+    """dominant FFT pass kernel by total time: algorithmic bytes per launch /
+    mean launch duration, against the MI355X HBM peak"""
+    fft = {k: v for k, v in rep.items() if k.startswith("fft_")}
+    name = max(fft, key=lambda k: fft[k]["ms_total"])
+    v = fft[name]
+    avg_ms = v["ms_total"] / v["count"]
+    bytes_per = v["bytes_total"] / v["count"]
+    gbs = bytes_per / (avg_ms * 1e-3) / 1e9
+    tot = sum(x["ms_total"] for x in rep.values())
+    fft_tot = sum(x["ms_total"] for x in fft.values())
+    fft_bytes = sum(x["bytes_total"] for x in fft.values())
+    return {
+        "kernel": name, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+        "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per, "launches_per_step": v["count"] / steps,
+        "all_fft_passes_GBs": round(fft_bytes / (fft_tot * 1e-3) / 1e9, 1),
+        "fft_time_fraction": round(fft_tot / tot, 3),
+    }
+
+
+def kernel_table(rep, steps):
+    out = {}
+    for k, v in sorted(rep.items(), key=lambda kv: -kv[1]["ms_total"]):
+        ms = v["ms_total"] / steps
+        out[k] = {"ms_per_step": round(ms, 3), "launches_per_step": v["count"] / steps,
+                  "GBs": round(v["bytes_total"] / (v["ms_total"] * 1e-3) / 1e9, 1) if v["ms_total"] > 0 else None}
+    return out
+
+
+def cpu_baseline(n_sample=128):
+    """CPU FP64 oracle (C + OpenMP, own FFT) on the same configuration at
+    n_sample^3 with the same cell size; PLT table build excluded as on the GPU."""
+    from oracle import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    box = 1000.0 * n_sample / 512
+    cfg = O.ic_cfg(n_sample, box, 3, 12345, lattice=1, plt=1)
+    oc = O.baseline_cosmology(24.0)
+    _, _, _, st = O.ic_run(cfg, oc, particles=True)
+    t = st[4] - st[2]
+    return {"value": round(n_sample ** 3 / t, 1), "unit": "cells/s", "cores": cores, "kind": "port",
+            "sample": f"oracle (C/OpenMP FP64, own FFT, reference schedule of 33 Orszag products) 3LPT+PLT bcc "
+                      f"{n_sample}^3 box {box:g} Mpc/h z=24 (same cell size as 512^3/1000); PLT table build "
+                      f"excluded; {t:.2f} s",
+            "seconds": round(t, 3)}
+
+
+def accuracy(ctx, n, box):
+    """FP32 vs FP64 device component fields (same seed): rel-L2 and max error"""
+    import numpy as np
+    from monofonic_amd import F32, F64, _lib as L
+    from monofonic_amd.lpt import ICConfig, ICRun
+    res = {}
+    fields = {}
+    for prec in (F64, F32):
+        cfg = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=L.BCC, plt=True, prec=prec, seed=12345,
+                       particles=False)
+        r = ICRun(ctx, cfg)
+        r.execute()
+        fields[prec] = [r.field(c).download(np.float64)[:, :, :n] for c in range(6)]
+        r.close()
+    for c, name in enumerate(("dx", "dy", "dz", "vx", "vy", "vz")):
+        a, b = fields[F32][c], fields[F64][c]
+        d = a - b
+        res[name] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(b)),
+                     "max_abs_rel": float(np.abs(d).max() / np.abs(b).max())}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box of this workload fits one "
+              "MI355X and multi-GPU runs are out of scope for this build.", file=sys.stderr)
+        sys.exit(2)
+
+    from monofonic_amd import BCC, F32, F64, Context
+    from monofonic_amd.lpt import ICConfig, ICRun
+
+    n, box = args.n, 1000.0 * args.n / 512
+    ctx = Context(0)
+    log(f"plan: 3LPT+PLT bcc n={n} box={box} FP32")
+    cfg = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=BCC, plt=True, prec=F32, seed=12345,
+                   particles=True)
+    run = ICRun(ctx, cfg)
+    log(f"timing {args.steps} steps after {args.warmup} warmup")
+    ms, rep = time_steps(ctx, run, args.steps, args.warmup, profile=True)
+    run.close()
+    log(f"FP32: {ms:.2f} ms/step")
+    line = {
+        "metric": METRIC, "value": round(n ** 3 / (ms * 1e-3), 1), "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (NGenIC white noise, seed 12345; Eisenstein-Hu P(k), sigma_8=0.81)",
+        "config": {"workload": f"3LPT+PLT bcc {n}^3 ({3 * n // 2}^3 padded), box {box:g} Mpc/h, z_start 24, "
+                               "FP32 bulk, FP64 q+Psi", "n": n, "padded": 3 * n // 2, "box": box, "zstart": 24.0,
+                   "lpt_order": 3, "lattice": "bcc", "plt": True, "particles": 2 * n ** 3},
+        "roofline": roofline(rep, args.steps),
+        "kernels": kernel_table(rep, args.steps),
+    }
+    if not args.no_fp64:
+        log("FP64 form of the same workload")
+        cfg64 = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=BCC, plt=True, prec=F64, seed=12345,
+                         particles=True)
+        run64 = ICRun(ctx, cfg64)
+        ms64, rep64 = time_steps(ctx, run64, max(2, args.steps // 2), 1, profile=True)
+        run64.close()
+        line["fp64"] = {"ms_per_step": round(ms64, 3), "value": round(n ** 3 / (ms64 * 1e-3), 1),
+                        "fp32_speedup": round(ms64 / ms, 3), "roofline": roofline(rep64, max(2, args.steps // 2))}
+        log(f"FP64: {ms64:.2f} ms/step")
+    if not args.no_accuracy:
+        log("accuracy: FP32 vs FP64 device fields")
+        line["accuracy"] = {"reference": "FP64 device run (same seed, same schedule)", **accuracy(ctx, n, box)}
+    ctx.close()
+    if not args.no_cpu:
+        log("CPU baseline (oracle) on a 128^3 sample")
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

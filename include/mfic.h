@@ -64,6 +64,13 @@ int mfic_ctx_destroy(mfic_ctx ctx);
 int mfic_ctx_sync(mfic_ctx ctx);
 /* raw hipStream_t of the context (for event timing / interop) */
 int mfic_ctx_stream(mfic_ctx ctx, void **stream);
+/* per-kernel timing with HIP events recorded on the context stream around
+ * every launch (the reference's per-FFT debug timings, grid_fft.cc:220-227).
+ * mfic_prof_report synchronises, writes a JSON object
+ * {kernel: {count, ms_total, bytes_total}} (bytes = algorithmic bytes) and
+ * clears the records. */
+int mfic_prof_enable(mfic_ctx ctx, int on);
+int mfic_prof_report(mfic_ctx ctx, char *buf, size_t len);
 
 /* ---------------------------------------------------------------- grid --- */
 /* Grid_FFT<real_t>(N, L, allocate=true, space) + allocate()

@@ -6,7 +6,7 @@ mirror of the reference's Grid_FFT / OrszagConvolver / ic_generator
 interfaces.  There is no CPU fallback for any device operation.
 """
 from . import _lib
-from ._lib import F32, F64, KSPACE, RSPACE, MficError
+from ._lib import BCC, F32, F64, KSPACE, RSPACE, SC, MficError
 from .grid import Context, Grid
 
-__all__ = ["Context", "Grid", "F32", "F64", "KSPACE", "RSPACE", "MficError", "_lib"]
+__all__ = ["Context", "Grid", "F32", "F64", "KSPACE", "RSPACE", "SC", "BCC", "MficError", "_lib"]

@@ -74,6 +74,8 @@ SIGNATURES = {
     "mfic_ctx_destroy": (C.c_int, [_P]),
     "mfic_ctx_sync": (C.c_int, [_P]),
     "mfic_ctx_stream": (C.c_int, [_P, C.POINTER(_P)]),
+    "mfic_prof_enable": (C.c_int, [_P, C.c_int]),
+    "mfic_prof_report": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "mfic_grid_create": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(_P)]),
     "mfic_grid_destroy": (C.c_int, [_P]),
     "mfic_grid_set_space": (C.c_int, [_P, C.c_int]),

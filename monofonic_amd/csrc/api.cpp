@@ -45,6 +45,29 @@ void* ctx_scratch(mfic_ctx ctx, const std::string& key, size_t bytes) {
     return s.first;
 }
 
+static hipEvent_t pool_event(mfic_ctx ctx) {
+    if (!ctx->prof_pool.empty()) {
+        hipEvent_t e = ctx->prof_pool.back();
+        ctx->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    MFIC_HIP(hipEventCreate(&e));
+    return e;
+}
+
+ProfScope::ProfScope(mfic_ctx ctx, const std::string& name, double bytes) : ctx_(ctx) {
+    if (!ctx_ || !ctx_->prof) return;
+    mfic_prof_rec r{name, bytes, pool_event(ctx_), pool_event(ctx_)};
+    MFIC_HIP(hipEventRecord(r.a, ctx_->stream));
+    idx_ = (long)ctx_->prof_recs.size();
+    ctx_->prof_recs.push_back(r);
+}
+
+ProfScope::~ProfScope() {
+    if (idx_ >= 0) (void)hipEventRecord(ctx_->prof_recs[idx_].b, ctx_->stream);
+}
+
 static void check_grid(mfic_grid g) {
     MFIC_CHECK(g != nullptr && g->d != nullptr, MFIC_ERR_ARG, "null or unallocated grid");
 }
@@ -83,6 +106,8 @@ int mfic_ctx_destroy(mfic_ctx ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         for (auto& kv : ctx->tables) (void)hipFree(kv.second.dev);
         for (auto& kv : ctx->scratch) (void)hipFree(kv.second.first);
+        for (auto& r : ctx->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        for (auto& e : ctx->prof_pool) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -99,6 +124,45 @@ int mfic_ctx_stream(mfic_ctx ctx, void** stream) {
     return guard([&] {
         MFIC_CHECK(ctx && stream, MFIC_ERR_ARG, "null argument");
         *stream = (void*)ctx->stream;
+    });
+}
+
+int mfic_prof_enable(mfic_ctx ctx, int on) {
+    return guard([&] {
+        MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
+        ctx->prof = on != 0;
+    });
+}
+
+int mfic_prof_report(mfic_ctx ctx, char* buf, size_t len) {
+    return guard([&] {
+        MFIC_CHECK(ctx && buf && len > 0, MFIC_ERR_ARG, "bad argument");
+        MFIC_HIP(hipStreamSynchronize(ctx->stream));
+        struct Agg { long count = 0; double ms = 0, bytes = 0; };
+        std::map<std::string, Agg> agg;
+        for (auto& r : ctx->prof_recs) {
+            float ms = 0.f;
+            MFIC_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            Agg& a = agg[r.name];
+            a.count += 1;
+            a.ms += ms;
+            a.bytes += r.bytes;
+            ctx->prof_pool.push_back(r.a);
+            ctx->prof_pool.push_back(r.b);
+        }
+        ctx->prof_recs.clear();
+        std::string out = "{";
+        bool first = true;
+        char tmp[512];
+        for (auto& kv : agg) {
+            snprintf(tmp, sizeof tmp, "%s\"%s\": {\"count\": %ld, \"ms_total\": %.6f, \"bytes_total\": %.0f}",
+                     first ? "" : ", ", kv.first.c_str(), kv.second.count, kv.second.ms, kv.second.bytes);
+            out += tmp;
+            first = false;
+        }
+        out += "}";
+        MFIC_CHECK(out.size() + 1 <= len, MFIC_ERR_ARG, "report buffer too small (need " + std::to_string(out.size() + 1) + ")");
+        std::memcpy(buf, out.c_str(), out.size() + 1);
     });
 }
 

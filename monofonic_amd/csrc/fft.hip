@@ -210,6 +210,8 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
     MFIC_CHECK(nouter <= 65535, MFIC_ERR_SIZE, "too many pencil groups");
     dim3 grid((tlen + tk - 1) / tk, (unsigned)nouter);
     size_t lds = (size_t)L * tk * sizeof(cx<T>);
+    ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + (FWD ? "_fwd" : "_inv") + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   2.0 * (double)tlen * L * nouter * sizeof(cx<T>));
     switch (tk) {
 #define TKCASE(K)                                                                                            \
     case K:                                                                                                  \
@@ -250,6 +252,8 @@ static void launch_rows_M(mfic_ctx ctx, T* data, size_t nrows, long pitch, T sca
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
     size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
+    ProfScope prof(ctx, std::string(R2C ? "fft_rows_r2c_n" : "fft_rows_c2r_n") + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   (double)nrows * (2.0 * M + 2.0 * (M + 1)) * sizeof(T));
     switch (rw) {
 #define RWCASE(K)                                                                                             \
     case K:                                                                                                   \

@@ -113,6 +113,7 @@ void grid_scale(mfic_grid g, double s) {
 
 void grid_fill(mfic_grid g, double v) {
     const int bs = 256;
+    ProfScope prof(g->ctx, "fill", (double)g->bytes());
     if (g->prec == MFIC_F64)
         hipLaunchKernelGGL(k_fill<double>, dim3(nblocks(g->nreal, bs)), dim3(bs), 0, g->ctx->stream, dptr<double>(g), g->nreal, v);
     else
@@ -144,6 +145,7 @@ void grid_inverse_laplacian(mfic_grid g) {
     const KGeom G = geom_of(g);
     const size_t nc = (size_t)G.n0 * G.n1 * G.nc;
     const int bs = 256;
+    ProfScope prof(g->ctx, "inverse_laplacian", 2.0 * g->bytes());
     if (g->prec == MFIC_F64)
         hipLaunchKernelGGL(k_inv_laplacian<double>, dim3(nblocks(nc, bs)), dim3(bs), 0, g->ctx->stream, (cx<double>*)g->d, G, nc);
     else
@@ -167,6 +169,7 @@ void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double vol
     const KGeom G = geom_of(phi);
     const size_t nc = (size_t)G.n0 * G.n1 * G.nc;
     const int bs = 256;
+    ProfScope prof(phi->ctx, "phi1", 2.0 * phi->bytes());
     if (phi->prec == MFIC_F64)
         hipLaunchKernelGGL(k_phi1<double>, dim3(nblocks(nc, bs)), dim3(bs), 0, phi->ctx->stream, (cx<double>*)phi->d,
                            (const cx<double>*)wn->d, G, nc, eh, volfac, fixing, inversion);

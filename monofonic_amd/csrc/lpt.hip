@@ -167,6 +167,8 @@ void assemble_component(mfic_ctx ctx, int prec, const size_t n[3], double L, int
     a.sh2 = shift_dx ? shift_dx[2] : 0.0;
     a.plt = plt;
     const size_t ntot = n[0] * n[1] * (n[2] / 2 + 1);
+    ProfScope prof(ctx, plt.Dr ? "assemble_plt" : "assemble",
+                   (double)ntot * (prec == MFIC_F64 ? 16 : 8) * (order + (order > 2 ? 2 : 0) + 1));
     if (prec == MFIC_F64)
         hipLaunchKernelGGL(k_assemble<double>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, (cx<double>*)out, a, L, ntot);
     else
@@ -177,6 +179,7 @@ void assemble_component(mfic_ctx ctx, int prec, const size_t n[3], double L, int
 void lattice_readout_raw(mfic_ctx ctx, int prec, const void* field, int n, int idim, int is_pos, double lunit,
                          double shift, size_t pbase, double* dst) {
     const size_t np = (size_t)n * n * n;
+    ProfScope prof(ctx, "readout", (double)np * ((prec == MFIC_F64 ? 8 : 4) + 8));
     if (prec == MFIC_F64)
         hipLaunchKernelGGL(k_readout<double>, dim3(nblk(np, 256)), dim3(256), 0, ctx->stream, (const double*)field, dst, n,
                            idim, is_pos, lunit, shift, pbase);

@@ -45,6 +45,12 @@ struct TwTable {
 }  // namespace mfic
 
 // ------------------------------------------------------------- handles ----
+struct mfic_prof_rec {
+    std::string name;
+    double bytes;  // algorithmic bytes of the launch
+    hipEvent_t a, b;
+};
+
 struct mfic_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -53,6 +59,10 @@ struct mfic_ctx_s {
     std::mutex mu;
     // scratch buffers owned by the context (reused across calls)
     std::map<std::string, std::pair<void*, size_t>> scratch;
+    // per-launch event timing (mfic_prof_enable)
+    bool prof = false;
+    std::vector<mfic_prof_rec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
 };
 
 struct mfic_grid_s {
@@ -70,6 +80,20 @@ struct mfic_grid_s {
 };
 
 namespace mfic {
+
+// Records HIP events around the launches in its scope when profiling is on
+// (same stream as the kernels, so only the bracketed kernels are timed).
+class ProfScope {
+  public:
+    ProfScope(mfic_ctx ctx, const std::string& name, double bytes);
+    ~ProfScope();
+    ProfScope(const ProfScope&) = delete;
+    ProfScope& operator=(const ProfScope&) = delete;
+
+  private:
+    mfic_ctx ctx_;
+    long idx_ = -1;
+};
 
 // scratch buffer of at least `bytes`, keyed by name, owned by the context
 void* ctx_scratch(mfic_ctx ctx, const std::string& key, size_t bytes);

@@ -303,6 +303,7 @@ void ngenic_fill(mfic_grid g, const uint32_t* dev_table) {
     const int n = (int)g->n[0];
     const long cols = (long)n * n;
     const int bs = 64;
+    ProfScope prof(g->ctx, "ngenic_fill", (double)cols * (n / 2 + 1) * (g->prec == MFIC_F64 ? 16 : 8));
     if (g->prec == MFIC_F64)
         hipLaunchKernelGGL(k_ngenic_fill<double>, dim3((unsigned)((cols + bs - 1) / bs)), dim3(bs), 0, g->ctx->stream,
                            (cx<double>*)g->d, dev_table, n, tabs);

@@ -12,6 +12,7 @@
 //    trunc(trunc(a b) c) (convolution.hh:525-532) with the inner products
 //    grouped by their outer factor.  Equal to the reference schedule in exact
 //    arithmetic (every step is linear in the grouped quantity).
+#include <algorithm>
 #include <cmath>
 
 #include "fft_core.hpp"
@@ -170,6 +171,7 @@ static void pad_insert_t(const ConvCore& c, const OperandDesc& o, void* fp) {
     DevOperand<T> od{static_cast<const cx<T>*>(o.f), o.kind, o.d[0], o.d[1], o.d[2], o.d[3]};
     const size_t ntot = (size_t)D.p0 * D.p1 * D.pc;
     const double rfac = std::pow(1.5, 1.5);
+    ProfScope prof(c.ctx, "pad_insert", (double)ntot * sizeof(cx<T>) + (double)D.n0 * D.n1 * D.nc * sizeof(cx<T>));
     hipLaunchKernelGGL(k_pad_insert<T>, dim3(nblk(ntot, 256)), dim3(256), 0, c.ctx->stream, (cx<T>*)fp, od, D, rfac, ntot);
     MFIC_LAUNCH_CHECK();
 }
@@ -179,6 +181,7 @@ static void unpad_t(const ConvCore& c, const void* fp, void* out, int op, double
     Dims D = dims_of(c.N, c.len);
     const size_t ntot = (size_t)D.n0 * D.n1 * D.nc;
     const double rfac = std::sqrt((double)(c.Np[0] * c.Np[1] * c.Np[2])) / std::sqrt((double)(c.N[0] * c.N[1] * c.N[2]));
+    ProfScope prof(c.ctx, "unpad", (double)ntot * sizeof(cx<T>) * (op == MFIC_OP_ASSIGN ? 2.0 : 3.0));
     hipLaunchKernelGGL(k_unpad<T>, dim3(nblk(ntot, 256)), dim3(256), 0, c.ctx->stream, (const cx<T>*)fp, (cx<T>*)out, D,
                        1.0 / rfac, op, coef, ntot);
     MFIC_LAUNCH_CHECK();
@@ -204,6 +207,7 @@ void ConvCore::r2c_padded(void* fp) const {
 }
 void ConvCore::mul(void* a, const void* b) const {
     const size_t n = padded_reals();
+    ProfScope prof(ctx, "mul", 3.0 * n * esize());
     if (prec == MFIC_F64)
         hipLaunchKernelGGL(k_mul<double>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (double*)a, (const double*)b, n);
     else
@@ -216,6 +220,11 @@ void ConvCore::prodsum(void* out, const std::vector<Term>& terms) const {
     P.n = (int)terms.size();
     for (size_t t = 0; t < terms.size(); ++t) { P.a[t] = terms[t].a; P.b[t] = terms[t].b; P.c[t] = terms[t].c; }
     const size_t n = padded_reals();
+    std::vector<const void*> distinct;
+    for (const Term& t : terms)
+        for (const void* p : {t.a, t.b})
+            if (std::find(distinct.begin(), distinct.end(), p) == distinct.end()) distinct.push_back(p);
+    ProfScope prof(ctx, "prodsum_" + std::to_string(terms.size()), (double)(distinct.size() + 1) * n * esize());
     if (prec == MFIC_F64)
         hipLaunchKernelGGL(k_prodsum<double>, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, (double*)out, P, n);
     else
