@@ -111,6 +111,12 @@ int mfic_grid_shift(mfic_grid g, const double s[3], int transform_back);
 int mfic_fft_forward(mfic_grid g);
 int mfic_fft_backward(mfic_grid g);
 int mfic_fft_size_supported(size_t n);
+/* autotuner hook: override the tile (pencils per block for kind 0 strided
+ * passes, rows per block for kind 1 contiguous-axis passes) and the LDS pitch
+ * padding for one transform length and precision; tile <= 0 restores the
+ * built-in choice.  No reference counterpart (FFTW planning,
+ * grid_fft.cc:53-54, plays this role there). */
+int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, int lds_pad);
 
 /* --------------------------------------------------------------- noise --- */
 /* RNG_ngenic constructor seed table (random_ngenic.cc:39-75), host only:

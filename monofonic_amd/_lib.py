@@ -92,6 +92,7 @@ SIGNATURES = {
     "mfic_fft_forward": (C.c_int, [_P]),
     "mfic_fft_backward": (C.c_int, [_P]),
     "mfic_fft_size_supported": (C.c_int, [C.c_size_t]),
+    "mfic_fft_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "mfic_ngenic_seedtable": (C.c_int, [C.c_long, C.c_size_t, _P]),
     "mfic_ngenic_fill": (C.c_int, [_P, C.c_long]),
     "mfic_cosmo_defaults": (None, [C.POINTER(CosmoParams)]),

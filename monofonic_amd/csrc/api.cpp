@@ -336,6 +336,20 @@ int mfic_fft_backward(mfic_grid g) {
 
 int mfic_fft_size_supported(size_t n) { return fft_length_supported(n) ? 1 : 0; }
 
+int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, int lds_pad) {
+    return guard([&] {
+        MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
+        MFIC_CHECK(fft_length_supported(length), MFIC_ERR_SIZE, "unsupported FFT length");
+        MFIC_CHECK(kind == 0 || kind == 1, MFIC_ERR_ARG, "kind must be 0 (strided pencils) or 1 (rows)");
+        MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
+        const auto key = std::make_tuple(length, kind, prec);
+        if (tile <= 0) { ctx->tiles.erase(key); return; }
+        MFIC_CHECK((tile & (tile - 1)) == 0 && tile <= 64, MFIC_ERR_ARG, "tile must be a power of two <= 64");
+        MFIC_CHECK(lds_pad >= 0 && lds_pad <= 8, MFIC_ERR_ARG, "lds_pad must be in [0, 8]");
+        ctx->tiles[key] = {tile, lds_pad};
+    });
+}
+
 void mfic_cosmo_defaults(mfic_cosmo_params* p) {
     if (p) cosmo_defaults(p);
 }

@@ -19,11 +19,10 @@ template <typename F> int guard_call(F&& f);
 
 struct mfic_conv_s {
     std::unique_ptr<mfic::ConvCore> core;
-    void *f1p = nullptr, *f2p = nullptr, *fbuf = nullptr;
+    void *kbuf = nullptr, *r1 = nullptr, *r2 = nullptr, *fbuf = nullptr;
     ~mfic_conv_s() {
-        if (f1p) (void)hipFree(f1p);
-        if (f2p) (void)hipFree(f2p);
-        if (fbuf) (void)hipFree(fbuf);
+        for (void* p : {kbuf, r1, r2, fbuf})
+            if (p) (void)hipFree(p);
     }
 };
 
@@ -39,7 +38,7 @@ struct mfic_run_s {
     mfic::EhParams eh{};
     mfic_lpt_coef coef{};
     mfic_grid pot[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // phi1 phi2 phi3 A3x A3y A3z
-    mfic_grid tmp = nullptr;
+    mfic_grid comp[12] = {};  // particle mode: component fields per sub-lattice
     mfic_grid fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t* d_table = nullptr;
     double *d_pos = nullptr, *d_vel = nullptr;
@@ -129,7 +128,7 @@ using namespace mfic;
 mfic_run_s::~mfic_run_s() {
     for (auto& g : pot) if (g) mfic_grid_destroy(g);
     for (auto& g : fields) if (g) mfic_grid_destroy(g);
-    if (tmp) mfic_grid_destroy(tmp);
+    for (auto& g : comp) if (g) mfic_grid_destroy(g);
     if (d_table) (void)hipFree(d_table);
     if (d_pos) (void)hipFree(d_pos);
     if (d_vel) (void)hipFree(d_vel);
@@ -164,10 +163,11 @@ int mfic_conv_create(mfic_ctx ctx, const size_t n[3], const double L[3], int pre
         MFIC_CHECK(ctx && n && L && out, MFIC_ERR_ARG, "null argument");
         auto c = std::make_unique<mfic_conv_s>();
         c->core = std::make_unique<ConvCore>(ctx, n, L, prec);
-        const size_t pb = c->core->padded_reals() * c->core->esize();
-        c->f1p = dmalloc(pb, "convolver padded buffer");
-        c->f2p = dmalloc(pb, "convolver padded buffer");
-        c->fbuf = dmalloc(c->core->grid_reals() * c->core->esize(), "convolver N-grid buffer");
+        const size_t es = c->core->esize();
+        c->kbuf = dmalloc(c->core->kbuf_reals() * es, "convolver padded k buffer");
+        c->r1 = dmalloc(c->core->real_reals() * es, "convolver padded real buffer");
+        c->r2 = dmalloc(c->core->real_reals() * es, "convolver padded real buffer");
+        c->fbuf = dmalloc(c->core->grid_reals() * es, "convolver N-grid buffer");
         *out = c.release();
     });
 }
@@ -194,7 +194,7 @@ int mfic_conv_convolve2(mfic_conv c, const mfic_operand* a, const mfic_operand* 
         MFIC_CHECK(op >= MFIC_OP_ASSIGN && op <= MFIC_OP_MADD, MFIC_ERR_ARG, "bad output operator");
         const OperandDesc da = to_desc(a), db = to_desc(b);
         MFIC_CHECK(a->f->prec == c->core->prec && b->f->prec == c->core->prec, MFIC_ERR_ARG, "precision mismatch");
-        c->core->convolve2(da, db, out->d, op, coef, c->f1p, c->f2p);
+        c->core->convolve2(da, db, out->d, op, coef, c->kbuf, c->r1, c->r2);
         out->space = MFIC_KSPACE;
     });
 }
@@ -205,7 +205,7 @@ int mfic_conv_convolve3(mfic_conv c, const mfic_operand* a, const mfic_operand* 
         conv_check_out(c, out);
         MFIC_CHECK(op >= MFIC_OP_ASSIGN && op <= MFIC_OP_MADD, MFIC_ERR_ARG, "bad output operator");
         const OperandDesc da = to_desc(a), db = to_desc(b), dc = to_desc(cc);
-        c->core->convolve3(da, db, dc, out->d, op, coef, c->f1p, c->f2p, c->fbuf);
+        c->core->convolve3(da, db, dc, out->d, op, coef, c->kbuf, c->r1, c->r2, c->fbuf);
         out->space = MFIC_KSPACE;
     });
 }
@@ -320,7 +320,7 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
         for (int p = 0; p < (cfg->lpt_order == 1 ? 1 : cfg->lpt_order == 2 ? 2 : 6); ++p)
             r->pot[p] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
         if (cfg->particles) {
-            r->tmp = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
+            for (int q = 0; q < 6 * (1 << cfg->lattice); ++q) r->comp[q] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
             r->npart = (size_t(1) << cfg->lattice) * n * n * n;
             r->d_pos = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "particle positions"));
             r->d_vel = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "particle velocities"));
@@ -366,25 +366,27 @@ int mfic_run_execute(mfic_run r, int blocking) {
         if (r->has_plt) pv = r->plt.view;
         const size_t nn[3] = {c.n, c.n, c.n};
         const double norm = 1.0 / std::sqrt((double)c.n * (double)c.n * (double)c.n);
-        for (int comp = 0; comp < 6; ++comp) {
-            double cf[4];
-            component_coefs(r->coef, comp, cf);
-            const double unit = comp < 3 ? c.lunit / c.L : c.vunit / c.L;
-            const int nsub = c.particles ? (1 << c.lattice) : 1;
-            for (int sub = 0; sub < nsub; ++sub) {
-                mfic_grid out = c.particles ? r->tmp : r->fields[comp];
-                const double dxs = 0.5 * c.L / (double)c.n;
-                const double shift[3] = {dxs, dxs, dxs};
-                assemble_component(r->ctx, c.prec, nn, c.L, c.lpt_order, comp, cf, unit, pv, phi, A3, out->d,
-                                   sub ? shift : nullptr);
-                if (c.prec == MFIC_F64) fft_c2r<double>(r->ctx, (double*)out->d, nn, norm);
-                else fft_c2r<float>(r->ctx, (float*)out->d, nn, norm);
-                out->space = MFIC_RSPACE;
-                if (c.particles) {
-                    double* dst = comp < 3 ? r->d_pos : r->d_vel;
-                    lattice_readout_raw(r->ctx, c.prec, out->d, (int)c.n, comp % 3, comp < 3, c.lunit,
-                                        sub ? 0.5 : 0.0, sub * c.n * c.n * c.n, dst);
-                }
+        double cd[4], cv[4];
+        component_coefs(r->coef, 0, cd);
+        component_coefs(r->coef, 3, cv);
+        const int nsub = c.particles ? (1 << c.lattice) : 1;
+        void* outs[12] = {};
+        mfic_grid* og = c.particles ? r->comp : r->fields;
+        for (int q = 0; q < 6 * nsub; ++q) outs[q] = og[q]->d;
+        assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3, outs,
+                     nsub);
+        for (int q = 0; q < 6 * nsub; ++q) {
+            if (c.prec == MFIC_F64) fft_c2r<double>(r->ctx, (double*)og[q]->d, nn, norm);
+            else fft_c2r<float>(r->ctx, (float*)og[q]->d, nn, norm);
+            og[q]->space = MFIC_RSPACE;
+        }
+        if (c.particles) {
+            for (int s = 0; s < nsub; ++s) {
+                const void* fp[3] = {og[6 * s]->d, og[6 * s + 1]->d, og[6 * s + 2]->d};
+                const void* fv[3] = {og[6 * s + 3]->d, og[6 * s + 4]->d, og[6 * s + 5]->d};
+                const size_t pbase = (size_t)s * c.n * c.n * c.n;
+                lattice_readout3(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, 0.5 * s, pbase, r->d_pos);
+                lattice_readout3(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, 0.5 * s, pbase, r->d_vel);
             }
         }
         if (blocking) MFIC_HIP(hipStreamSynchronize(r->ctx->stream));

@@ -22,7 +22,7 @@ namespace mfic {
 template <typename T, int L, int TK, bool FWD>
 __global__ void __launch_bounds__(Plan<L>::P * TK)
 k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
-          const cx<T>* __restrict__ tw) {
+          const cx<T>* __restrict__ tw, int tkp) {
     constexpr int E = Plan<L>::E, P = Plan<L>::P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
@@ -33,7 +33,7 @@ k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? base[(long)(q + P * e) * ps] : cx<T>{0, 0};
-    fft_regs<T, L, FWD>(v, q, lds, LdsMap{t, TK}, tw);
+    fft_regs<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) base[(long)(q + P * e) * ps] = cscale(v[e], scale);
@@ -198,26 +198,43 @@ static int pow2_floor(int x) {
     while (p * 2 <= x) p *= 2;
     return p;
 }
-static constexpr size_t kMaxLds = 65536;
+static size_t max_lds(mfic_ctx ctx) {
+    if (ctx->max_lds == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess || v <= 0)
+            v = 65536;
+        ctx->max_lds = (size_t)v;
+    }
+    return ctx->max_lds;
+}
 
 template <typename T, int L, bool FWD>
-static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tlen, long nouter, T scale) {
+static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tlen, long nouter, T scale,
+                             const char* tag) {
     constexpr int P = Plan<L>::P;
-    int tk = std::max(4, std::min(64, pow2_floor(256 / P)));
-    while (tk > 1 && (size_t)L * tk * sizeof(cx<T>) > kMaxLds) tk /= 2;
+    // widest tile (pencils per block) whose LDS fits ~136 KB: 128-byte row
+    // segments per pencil position and one resident block per CU measured
+    // best on MI355X for the 512 / 768 passes
+    int tk = 64;
+    while (tk > 4 && ((size_t)L * tk * sizeof(cx<T>) > 139264 || P * tk > 1024)) tk /= 2;
+    int pad = 0;
+    const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    if (ov != ctx->tiles.end()) { tk = ov->second.first; pad = ov->second.second; }
+    while (tk > 1 && (size_t)L * (tk + pad) * sizeof(cx<T>) > max_lds(ctx)) tk /= 2;
     while (tk > 1 && P * tk > 1024) tk /= 2;
+    const int tkp = tk + pad;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     MFIC_CHECK(nouter <= 65535, MFIC_ERR_SIZE, "too many pencil groups");
     dim3 grid((tlen + tk - 1) / tk, (unsigned)nouter);
-    size_t lds = (size_t)L * tk * sizeof(cx<T>);
-    ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + (FWD ? "_fwd" : "_inv") + (sizeof(T) == 8 ? "_f64" : "_f32"),
+    size_t lds = (size_t)L * tkp * sizeof(cx<T>);
+    ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + (FWD ? "_fwd" : "_inv") + tag + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    2.0 * (double)tlen * L * nouter * sizeof(cx<T>));
     switch (tk) {
 #define TKCASE(K)                                                                                            \
     case K:                                                                                                  \
         if constexpr (P * K <= 1024)                                                                         \
             hipLaunchKernelGGL((k_pencils<T, L, K, FWD>), grid, dim3(P * K), lds, ctx->stream, data, ps, os, \
-                               tlen, scale, tw);                                                             \
+                               tlen, scale, tw, tkp);                                                        \
         break;
         TKCASE(1) TKCASE(2) TKCASE(4) TKCASE(8) TKCASE(16) TKCASE(32) TKCASE(64)
 #undef TKCASE
@@ -229,13 +246,13 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
 
 template <typename T>
 void fft_pencils(mfic_ctx ctx, void* data, int L, long ps, long os, int tlen, long nouter, bool fwd,
-                 double scale) {
+                 double scale, const char* tag) {
     cx<T>* d = static_cast<cx<T>*>(data);
     switch (L) {
-#define X(S)                                                                         \
-    case S:                                                                          \
-        if (fwd) launch_pencils_L<T, S, true>(ctx, d, ps, os, tlen, nouter, (T)scale); \
-        else launch_pencils_L<T, S, false>(ctx, d, ps, os, tlen, nouter, (T)scale);    \
+#define X(S)                                                                                \
+    case S:                                                                                 \
+        if (fwd) launch_pencils_L<T, S, true>(ctx, d, ps, os, tlen, nouter, (T)scale, tag); \
+        else launch_pencils_L<T, S, false>(ctx, d, ps, os, tlen, nouter, (T)scale, tag);    \
         break;
         MFIC_FOR_SIZES(X)
 #undef X
@@ -247,8 +264,10 @@ void fft_pencils(mfic_ctx ctx, void* data, int L, long ps, long os, int tlen, lo
 template <typename T, int M, bool R2C>
 static void launch_rows_M(mfic_ctx ctx, T* data, size_t nrows, long pitch, T scale) {
     constexpr int P = Plan<M>::P;
-    int rw = std::max(1, std::min(64, pow2_floor(256 / P)));
-    while (rw > 1 && (size_t)(M + 1) * rw * sizeof(cx<T>) > kMaxLds) rw /= 2;
+    int rw = std::max(1, std::min(64, pow2_floor((R2C ? 512 : 256) / P / (M >= 256 ? 4 : 1))));
+    const auto ov = ctx->tiles.find(std::make_tuple(M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    if (ov != ctx->tiles.end()) rw = ov->second.first;
+    while (rw > 1 && (size_t)(M + 1) * rw * sizeof(cx<T>) > max_lds(ctx)) rw /= 2;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
     size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
@@ -306,15 +325,16 @@ template <typename T> void fft_r2c(mfic_ctx ctx, T* data, const size_t n[3], dou
     check_sizes(n);
     const long nc = (long)(n[2] / 2 + 1);
     fft_rows_r2c<T>(ctx, data, n[0] * n[1], (int)n[2], 1.0);
-    fft_pencils<T>(ctx, data, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], true, 1.0);
-    fft_pencils<T>(ctx, data, (int)n[0], (long)n[1] * nc, nc, (int)nc, (long)n[1], true, scale);
+    fft_pencils<T>(ctx, data, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], true, 1.0, "_ax1");
+    // axis 0: the (j, k) plane is contiguous, so tiles run over it flat
+    fft_pencils<T>(ctx, data, (int)n[0], (long)n[1] * nc, 0, (int)(n[1] * nc), 1, true, scale, "_ax0");
 }
 
 template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], double scale) {
     check_sizes(n);
     const long nc = (long)(n[2] / 2 + 1);
-    fft_pencils<T>(ctx, data, (int)n[0], (long)n[1] * nc, nc, (int)nc, (long)n[1], false, 1.0);
-    fft_pencils<T>(ctx, data, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], false, 1.0);
+    fft_pencils<T>(ctx, data, (int)n[0], (long)n[1] * nc, 0, (int)(n[1] * nc), 1, false, 1.0, "_ax0");
+    fft_pencils<T>(ctx, data, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], false, 1.0, "_ax1");
     fft_rows_c2r<T>(ctx, data, n[0] * n[1], (int)n[2], scale);
 }
 
@@ -326,7 +346,7 @@ template void fft_rows_r2c<float>(mfic_ctx, float*, size_t, int, double);
 template void fft_rows_r2c<double>(mfic_ctx, double*, size_t, int, double);
 template void fft_rows_c2r<float>(mfic_ctx, float*, size_t, int, double);
 template void fft_rows_c2r<double>(mfic_ctx, double*, size_t, int, double);
-template void fft_pencils<float>(mfic_ctx, void*, int, long, long, int, long, bool, double);
-template void fft_pencils<double>(mfic_ctx, void*, int, long, long, int, long, bool, double);
+template void fft_pencils<float>(mfic_ctx, void*, int, long, long, int, long, bool, double, const char*);
+template void fft_pencils<double>(mfic_ctx, void*, int, long, long, int, long, bool, double, const char*);
 
 }  // namespace mfic

@@ -61,10 +61,15 @@ __device__ inline void plt_grad(const PltDev& P, int n, double L, int i, int j, 
     const double kmod = kr / mr / L;
     const double Dr = cic_k(P.Dr, ng, ix, iy, iz), Dt = cic_k(P.Dtheta, ng, ix, iy, iz),
                  Dp = cic_k(P.Dphi, ng, ix, iy, iz);
-    const double kphi = kr > 0.0 ? atan2(ky, kx) : 0.0, kth = kr > 0.0 ? acos(kz / kr) : 0.0;
-    double st, ct, sp, cp;
-    sincos(kth, &st, &ct);
-    sincos(kphi, &sp, &cp);
+    // spherical basis of k without trigonometry: theta = acos(kz/|k|),
+    // phi = atan2(ky, kx) (phi = 0 on the kz axis, as atan2(0, 0))
+    double st = 0.0, ct = 1.0, sp = 0.0, cp = 1.0;
+    if (kr > 0.0) {
+        const double kp = sqrt(kx * kx + ky * ky);
+        ct = kz / kr;
+        st = kp / kr;
+        if (kp > 0.0) { cp = kx / kp; sp = ky / kp; }
+    }
     g[0] = kmod * (Dr * st * cp + Dt * ct * cp - Dp * sp);
     g[1] = kmod * (Dr * st * sp + Dt * ct * sp + Dp * cp);
     g[2] = kmod * (Dr * ct - Dt * st);
@@ -130,6 +135,122 @@ __global__ void k_assemble(cx<T>* __restrict__ out, AsmArgs a, double L, size_t 
     out[idx] = cx<T>{(T)vr, (T)vi};
 }
 
+// All six components (and, for bcc, their (1/2,1/2,1/2)-shifted copies) in
+// one pass: the potentials are read once, the gradient (Fourier or PLT) is
+// evaluated once per mode.  out[comp + 6*sub] (null pointers are skipped).
+struct AsmAllArgs {
+    const void* phi[3];
+    const void* A3[3];
+    void* out[12];
+    int n, nc, order, nsub;
+    double kf, L;
+    double c[4], cv[4];       // displacement / velocity coefficients
+    double ulen, uvel;        // lunit / L, vunit / L
+    double sh;                // sub-lattice shift (1/2) * dx
+    PltDev plt;
+};
+
+template <typename T>
+__global__ void k_assemble_all(AsmAllArgs a, size_t ntot) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntot) return;
+    const int k = (int)(idx % a.nc);
+    const size_t r = idx / a.nc;
+    const int j = (int)(r % a.n), i = (int)(r / a.n);
+    double g[3], vc = 1.0;
+    if (a.plt.Dr) {
+        plt_grad(a.plt, a.n, a.L, i, j, k, g, &vc);
+    } else {
+        g[0] = fgrad(i, a.n, a.kf);
+        g[1] = fgrad(j, a.n, a.kf);
+        g[2] = fgrad(k, a.n, a.kf);
+    }
+    double pr[3] = {0, 0, 0}, pim[3] = {0, 0, 0};
+    for (int o = 0; o < a.order; ++o) {
+        const cx<T> v = static_cast<const cx<T>*>(a.phi[o])[idx];
+        pr[o] = (double)v.x;
+        pim[o] = (double)v.y;
+    }
+    // compensated sums of c_o phi_o for displacement and velocity
+    auto csum = [&](const double* cf, double& sr, double& si) {
+        sr = si = 0.0;
+        double er = 0.0, ei = 0.0;
+        for (int o = 0; o < a.order; ++o) {
+            const double xr = cf[o] * pr[o], xi = cf[o] * pim[o];
+            double t = sr + xr, bb = t - sr;
+            er += (sr - (t - bb)) + (xr - bb);
+            sr = t;
+            t = si + xi; bb = t - si;
+            ei += (si - (t - bb)) + (xi - bb);
+            si = t;
+        }
+        sr += er;
+        si += ei;
+    };
+    double dr, di, vr, vi;
+    csum(a.c, dr, di);
+    csum(a.cv, vr, vi);
+    double Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
+    if (a.order > 2)
+        for (int d = 0; d < 3; ++d) {
+            const cx<T> v = static_cast<const cx<T>*>(a.A3[d])[idx];
+            Ar[d] = (double)v.x;
+            Ai[d] = (double)v.y;
+        }
+    double ph_c = 1.0, ph_s = 0.0;
+    if (a.nsub > 1) {
+        const double kx = ((double)i - (double)(i > a.n / 2) * a.n) * a.kf;
+        const double ky = ((double)j - (double)(j > a.n / 2) * a.n) * a.kf;
+        const double kz = ((double)k - (double)(k > a.n / 2) * a.n) * a.kf;
+        sincos(a.sh * kx + a.sh * ky + a.sh * kz, &ph_s, &ph_c);
+    }
+    for (int comp = 0; comp < 6; ++comp) {
+        const bool vel = comp >= 3;
+        const int d = comp % 3, dp = (d + 1) % 3, dpp = (d + 2) % 3;
+        const double sr = vel ? vr : dr, si = vel ? vi : di;
+        double xr = -g[d] * si, xi = g[d] * sr;
+        if (a.order > 2) {
+            const double cr = (-g[dp] * Ai[dpp]) - (-g[dpp] * Ai[dp]);
+            const double ci = (g[dp] * Ar[dpp]) - (g[dpp] * Ar[dp]);
+            const double cA = vel ? a.cv[3] : a.c[3];
+            xr += cA * cr;
+            xi += cA * ci;
+        }
+        if (vel) { xr *= vc; xi *= vc; }
+        const double u = vel ? a.uvel : a.ulen;
+        xr *= u;
+        xi *= u;
+        if (idx == 0) { xr = 0.0; xi = 0.0; }
+        if (a.out[comp]) static_cast<cx<T>*>(a.out[comp])[idx] = cx<T>{(T)xr, (T)xi};
+        if (a.nsub > 1 && a.out[comp + 6])
+            static_cast<cx<T>*>(a.out[comp + 6])[idx] = cx<T>{(T)(xr * ph_c - xi * ph_s), (T)(xr * ph_s + xi * ph_c)};
+    }
+}
+
+// three real-space components -> interleaved [ipart][3] FP64 (positions add
+// the lattice coordinate ((ijk + shift)/N) * lunit)
+template <typename T>
+__global__ void k_readout3(const T* __restrict__ f0, const T* __restrict__ f1, const T* __restrict__ f2,
+                           double* __restrict__ dst, int n, int is_pos, double lunit, double shift, size_t pbase) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t np = (size_t)n * n * n;
+    if (p >= np) return;
+    const int k = (int)(p % n);
+    const size_t r = p / n;
+    const int j = (int)(r % n), i = (int)(r / n);
+    const size_t src = r * (n + 2) + k;
+    double v0 = (double)f0[src], v1 = (double)f1[src], v2 = (double)f2[src];
+    if (is_pos) {
+        v0 = (((double)i + shift) / (double)n) * lunit + v0;
+        v1 = (((double)j + shift) / (double)n) * lunit + v1;
+        v2 = (((double)k + shift) / (double)n) * lunit + v2;
+    }
+    double* o = dst + 3 * (pbase + p);
+    o[0] = v0;
+    o[1] = v1;
+    o[2] = v2;
+}
+
 // particle value = q + field (positions, FP64) or field (velocities)
 template <typename T>
 __global__ void k_readout(const T* __restrict__ f, double* __restrict__ dst, int n, int idim, int is_pos, double lunit,
@@ -173,6 +294,43 @@ void assemble_component(mfic_ctx ctx, int prec, const size_t n[3], double L, int
         hipLaunchKernelGGL(k_assemble<double>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, (cx<double>*)out, a, L, ntot);
     else
         hipLaunchKernelGGL(k_assemble<float>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, (cx<float>*)out, a, L, ntot);
+    MFIC_LAUNCH_CHECK();
+}
+
+void assemble_all(mfic_ctx ctx, int prec, size_t n, double L, int order, const double c[4], const double cv[4],
+                  double ulen, double uvel, const PltDev& plt, const void* const phi[3], const void* const A3[3],
+                  void* const out[12], int nsub) {
+    AsmAllArgs a{};
+    for (int o = 0; o < 3; ++o) { a.phi[o] = phi[o]; a.A3[o] = A3[o]; }
+    int nout = 0;
+    for (int q = 0; q < 12; ++q) { a.out[q] = out[q]; nout += out[q] != nullptr; }
+    a.n = (int)n; a.nc = (int)(n / 2 + 1); a.order = order; a.nsub = nsub;
+    a.kf = 2.0 * 3.14159265358979323846 / L;
+    a.L = L;
+    for (int q = 0; q < 4; ++q) { a.c[q] = c[q]; a.cv[q] = cv[q]; }
+    a.ulen = ulen; a.uvel = uvel;
+    a.sh = 0.5 * L / (double)n;
+    a.plt = plt;
+    const size_t ntot = n * n * (n / 2 + 1);
+    ProfScope prof(ctx, plt.Dr ? "assemble_all_plt" : "assemble_all",
+                   (double)ntot * (prec == MFIC_F64 ? 16 : 8) * (order + (order > 2 ? 3 : 0) + nout));
+    if (prec == MFIC_F64)
+        hipLaunchKernelGGL(k_assemble_all<double>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, a, ntot);
+    else
+        hipLaunchKernelGGL(k_assemble_all<float>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, a, ntot);
+    MFIC_LAUNCH_CHECK();
+}
+
+void lattice_readout3(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit, double shift,
+                      size_t pbase, double* dst) {
+    const size_t np = (size_t)n * n * n;
+    ProfScope prof(ctx, "readout3", (double)np * 3 * ((prec == MFIC_F64 ? 8 : 4) + 8));
+    if (prec == MFIC_F64)
+        hipLaunchKernelGGL(k_readout3<double>, dim3(nblk(np, 256)), dim3(256), 0, ctx->stream, (const double*)f[0],
+                           (const double*)f[1], (const double*)f[2], dst, n, is_pos, lunit, shift, pbase);
+    else
+        hipLaunchKernelGGL(k_readout3<float>, dim3(nblk(np, 256)), dim3(256), 0, ctx->stream, (const float*)f[0],
+                           (const float*)f[1], (const float*)f[2], dst, n, is_pos, lunit, shift, pbase);
     MFIC_LAUNCH_CHECK();
 }
 

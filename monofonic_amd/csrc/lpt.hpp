@@ -33,6 +33,14 @@ void assemble_component(mfic_ctx ctx, int prec, const size_t n[3], double L, int
                         double unit, const PltDev& plt, const void* const phi[3], const void* const A3[3], void* out,
                         const double shift_dx[3]);
 
+// all 6 components (x nsub sub-lattices) in one pass; out[comp + 6 sub]
+void assemble_all(mfic_ctx ctx, int prec, size_t n, double L, int order, const double c[4], const double cv[4],
+                  double ulen, double uvel, const PltDev& plt, const void* const phi[3], const void* const A3[3],
+                  void* const out[12], int nsub);
+// three real-space component fields -> interleaved FP64 particle values
+void lattice_readout3(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit, double shift,
+                      size_t pbase, double* dst);
+
 void lattice_readout_raw(mfic_ctx ctx, int prec, const void* field, int n, int idim, int is_pos, double lunit,
                          double shift, size_t pbase, double* dst);
 

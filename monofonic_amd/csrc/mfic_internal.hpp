@@ -59,6 +59,9 @@ struct mfic_ctx_s {
     std::mutex mu;
     // scratch buffers owned by the context (reused across calls)
     std::map<std::string, std::pair<void*, size_t>> scratch;
+    size_t max_lds = 0;  // LDS bytes per workgroup (device attribute)
+    // tile overrides from the autotuner: (length, kind, prec) -> (tile, lds pad)
+    std::map<std::tuple<int, int, int>, std::pair<int, int>> tiles;
     // per-launch event timing (mfic_prof_enable)
     bool prof = false;
     std::vector<mfic_prof_rec> prof_recs;
@@ -114,6 +117,6 @@ template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int
 // t < tlen, o < nouter (offsets in complex elements)
 template <typename T>
 void fft_pencils(mfic_ctx ctx, void* data, int L, long ps, long os, int tlen, long nouter, bool fwd,
-                 double scale);
+                 double scale, const char* tag = "");
 
 }  // namespace mfic

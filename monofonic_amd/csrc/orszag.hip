@@ -19,6 +19,7 @@
 #include "kspace.hpp"
 #include "mfic_internal.hpp"
 #include "orszag.hpp"
+#include "pruned.hpp"
 
 namespace mfic {
 
@@ -232,23 +233,23 @@ void ConvCore::prodsum(void* out, const std::vector<Term>& terms) const {
     MFIC_LAUNCH_CHECK();
 }
 
-// convolve2 in the reference's schedule
-void ConvCore::convolve2(const OperandDesc& a, const OperandDesc& b, void* out, int op, double coef, void* f1p,
-                         void* f2p) const {
-    pad_insert(a, f1p);
-    pad_insert(b, f2p);
-    c2r_padded(f1p);
-    c2r_padded(f2p);
-    mul(f2p, f1p);
-    r2c_padded(f2p);
-    unpad(f2p, out, op, coef);
+// convolve2: both operands to padded real space (pruned c2r), product, pruned
+// r2c with the unpad rules and the output operator (convolution.hh:495-518)
+size_t ConvCore::kbuf_reals() const { return 2 * Np[0] * Np[1] * (N[2] / 2 + 1); }
+size_t ConvCore::real_reals() const { return Np[0] * Np[1] * Np[2]; }
+
+void ConvCore::convolve2(const OperandDesc& a, const OperandDesc& b, void* out, int op, double coef, void* kbuf,
+                         void* r1, void* r2) const {
+    padded_c2r_from_operand(ctx, prec, a, N[0], len[0], kbuf, r1);
+    padded_c2r_from_operand(ctx, prec, b, N[0], len[0], kbuf, r2);
+    padded_r2c_products(ctx, prec, {{r1, r2, 1.0}}, N[0], len[0], kbuf, out, op, coef, false);
 }
 
 void ConvCore::convolve3(const OperandDesc& a, const OperandDesc& b, const OperandDesc& c, void* out, int op,
-                         double coef, void* f1p, void* f2p, void* fbuf) const {
-    convolve2(a, b, fbuf, MFIC_OP_ASSIGN, 0.0, f1p, f2p);
+                         double coef, void* kbuf, void* r1, void* r2, void* fbuf) const {
+    convolve2(a, b, fbuf, MFIC_OP_ASSIGN, 0.0, kbuf, r1, r2);
     OperandDesc id{fbuf, MFIC_OPND_IDENT, {0, 0, 0, 0}};
-    convolve2(id, c, out, op, coef, f1p, f2p);
+    convolve2(id, c, out, op, coef, kbuf, r1, r2);
 }
 
 // ------------------------------------------------- reorganised sources ----
@@ -257,7 +258,12 @@ void lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_g
                  mfic_grid A3y, mfic_grid A3z) {
     if (order < 2) return;
     ConvCore C(ctx, phi1->n, phi1->L, phi1->prec);
-    const size_t pb = C.padded_reals() * C.esize();
+    MFIC_CHECK(phi1->n[0] == phi1->n[1] && phi1->n[1] == phi1->n[2], MFIC_ERR_SIZE, "LPT sources need a cubic grid");
+    const size_t n = phi1->n[0];
+    const double L = phi1->L[0];
+    const int prec = phi1->prec;
+    const size_t rb = C.real_reals() * C.esize();
+    void* kbuf = ctx_scratch(ctx, "kbuf", C.kbuf_reals() * C.esize());
     // xx yy zz xy xz yz
     static const int HD[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
     auto hidx = [](int a, int b) {
@@ -266,77 +272,62 @@ void lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_g
         return a == 0 ? (b == 1 ? 3 : 4) : 5;
     };
     void* H1[6];
-    for (int h = 0; h < 6; ++h) H1[h] = ctx_scratch(ctx, "H1_" + std::to_string(h), pb);
-    void* work = ctx_scratch(ctx, "pad_work", pb);
     for (int h = 0; h < 6; ++h) {
-        C.pad_insert(OperandDesc{phi1->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, H1[h]);
-        C.c2r_padded(H1[h]);
+        H1[h] = ctx_scratch(ctx, "H1_" + std::to_string(h), rb);
+        padded_c2r_from_operand(ctx, prec, OperandDesc{phi1->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
+                                kbuf, H1[h]);
     }
-    // phi2 source (ic_generator.cc:536-541)
-    {
-        using T = ConvCore::Term;
-        std::vector<T> t = {{H1[0], H1[1], 1.0}, {H1[0], H1[2], 1.0}, {H1[1], H1[2], 1.0},
-                            {H1[3], H1[3], -1.0}, {H1[4], H1[4], -1.0}, {H1[5], H1[5], -1.0}};
-        C.prodsum(work, t);
-        C.r2c_padded(work);
-        C.unpad(work, phi2->d, MFIC_OP_ASSIGN, 0.0);
-        phi2->space = MFIC_KSPACE;
-        grid_inverse_laplacian(phi2);
-    }
+    // phi2 source (ic_generator.cc:536-541), then -1/k^2 (ic_generator.cc:554)
+    padded_r2c_products(ctx, prec,
+                        {{H1[0], H1[1], 1.0}, {H1[0], H1[2], 1.0}, {H1[1], H1[2], 1.0},
+                         {H1[3], H1[3], -1.0}, {H1[4], H1[4], -1.0}, {H1[5], H1[5], -1.0}},
+                        n, L, kbuf, phi2->d, MFIC_OP_ASSIGN, 0.0, true);
+    phi2->space = MFIC_KSPACE;
     if (order < 3) return;
     void* H2[6];
-    for (int h = 0; h < 6; ++h) H2[h] = ctx_scratch(ctx, "H2_" + std::to_string(h), pb);
     for (int h = 0; h < 6; ++h) {
-        C.pad_insert(OperandDesc{phi2->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, H2[h]);
-        C.c2r_padded(H2[h]);
+        H2[h] = ctx_scratch(ctx, "H2_" + std::to_string(h), rb);
+        padded_c2r_from_operand(ctx, prec, OperandDesc{phi2->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
+                                kbuf, H2[h]);
     }
     // 3a inner products, truncated once and brought back to padded real space:
     // Qzz = xx yy - xy^2 (outer zz), Qyz = xy xz (outer yz, x2), Qxx = yz^2
     // (outer xx, -1), Qyy = xz^2 (outer yy, -1)   (ic_generator.cc:581-585)
-    using Tm = ConvCore::Term;
-    const std::vector<Tm> inner[4] = {{{H1[0], H1[1], 1.0}, {H1[3], H1[3], -1.0}},
-                                      {{H1[3], H1[4], 1.0}},
-                                      {{H1[5], H1[5], 1.0}},
-                                      {{H1[4], H1[4], 1.0}}};
+    const std::vector<ProdTerm> inner[4] = {{{H1[0], H1[1], 1.0}, {H1[3], H1[3], -1.0}},
+                                            {{H1[3], H1[4], 1.0}},
+                                            {{H1[5], H1[5], 1.0}},
+                                            {{H1[4], H1[4], 1.0}}};
     void* Q[4];
-    const size_t gb = C.grid_reals() * C.esize();
-    void* qk = ctx_scratch(ctx, "qk", gb);
+    void* qk = ctx_scratch(ctx, "qk", C.grid_reals() * C.esize());
     for (int q = 0; q < 4; ++q) {
-        Q[q] = ctx_scratch(ctx, "Q_" + std::to_string(q), pb);
-        C.prodsum(Q[q], inner[q]);
-        C.r2c_padded(Q[q]);
-        C.unpad(Q[q], qk, MFIC_OP_ASSIGN, 0.0);
-        C.pad_insert(OperandDesc{qk, MFIC_OPND_IDENT, {0, 0, 0, 0}}, Q[q]);
-        C.c2r_padded(Q[q]);
+        Q[q] = ctx_scratch(ctx, "Q_" + std::to_string(q), rb);
+        padded_r2c_products(ctx, prec, inner[q], n, L, kbuf, qk, MFIC_OP_ASSIGN, 0.0, false);
+        padded_c2r_from_operand(ctx, prec, OperandDesc{qk, MFIC_OPND_IDENT, {0, 0, 0, 0}}, n, L, kbuf, Q[q]);
     }
     // phi3 source: 3a outer products + 3b (ic_generator.cc:581-609)
     {
         const double a = -5.0 / 7.0, b = 10.0 / 7.0;
-        std::vector<Tm> t = {{Q[0], H1[2], 1.0}, {Q[1], H1[5], 2.0}, {Q[2], H1[0], -1.0}, {Q[3], H1[1], -1.0},
+        padded_r2c_products(ctx, prec,
+                            {{Q[0], H1[2], 1.0}, {Q[1], H1[5], 2.0}, {Q[2], H1[0], -1.0}, {Q[3], H1[1], -1.0},
                              {H1[0], H2[1], a}, {H1[0], H2[2], a}, {H1[1], H2[2], a}, {H1[1], H2[0], a},
                              {H1[2], H2[0], a}, {H1[2], H2[1], a},
-                             {H1[3], H2[3], b}, {H1[4], H2[4], b}, {H1[5], H2[5], b}};
-        C.prodsum(work, t);
-        C.r2c_padded(work);
-        C.unpad(work, phi3->d, MFIC_OP_ASSIGN, 0.0);
+                             {H1[3], H2[3], b}, {H1[4], H2[4], b}, {H1[5], H2[5], b}},
+                            n, L, kbuf, phi3->d, MFIC_OP_ASSIGN, 0.0, true);
         phi3->space = MFIC_KSPACE;
-        grid_inverse_laplacian(phi3);
     }
     // A3 (ic_generator.cc:614-627)
     mfic_grid A3[3] = {A3x, A3y, A3z};
     for (int d = 0; d < 3; ++d) {
         const int dp = (d + 1) % 3, dpp = (d + 2) % 3;
-        std::vector<Tm> t = {{H2[hidx(d, dp)], H1[hidx(d, dpp)], 1.0},
+        padded_r2c_products(ctx, prec,
+                            {{H2[hidx(d, dp)], H1[hidx(d, dpp)], 1.0},
                              {H2[hidx(d, dpp)], H1[hidx(d, dp)], -1.0},
                              {H1[hidx(dp, dpp)], H2[hidx(dp, dp)], 1.0},
                              {H1[hidx(dp, dpp)], H2[hidx(dpp, dpp)], -1.0},
                              {H2[hidx(dp, dpp)], H1[hidx(dp, dp)], -1.0},
-                             {H2[hidx(dp, dpp)], H1[hidx(dpp, dpp)], 1.0}};
-        C.prodsum(work, t);
-        C.r2c_padded(work);
-        C.unpad(work, A3[d]->d, MFIC_OP_ASSIGN, 0.0);
+                             {H2[hidx(dp, dpp)], H1[hidx(dpp, dpp)], 1.0}},
+                            n, L, kbuf, A3[d]->d, MFIC_OP_ASSIGN, 0.0, true);
         A3[d]->space = MFIC_KSPACE;
-        grid_inverse_laplacian(A3[d]);
     }
 }
 

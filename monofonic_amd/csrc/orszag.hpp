@@ -29,10 +29,14 @@ struct ConvCore {
     void r2c_padded(void* fp) const;
     void mul(void* a, const void* b) const;
     void prodsum(void* out, const std::vector<Term>& terms) const;
-    void convolve2(const OperandDesc& a, const OperandDesc& b, void* out, int op, double coef, void* f1p,
-                   void* f2p) const;
+    // buffer sizes (reals) of the pruned path: compact padded k buffer and a
+    // padded real field
+    size_t kbuf_reals() const;
+    size_t real_reals() const;
+    void convolve2(const OperandDesc& a, const OperandDesc& b, void* out, int op, double coef, void* kbuf, void* r1,
+                   void* r2) const;
     void convolve3(const OperandDesc& a, const OperandDesc& b, const OperandDesc& c, void* out, int op, double coef,
-                   void* f1p, void* f2p, void* fbuf) const;
+                   void* kbuf, void* r1, void* r2, void* fbuf) const;
 };
 
 void add_inplace(mfic_ctx ctx, int prec, void* a, const void* b, size_t n);

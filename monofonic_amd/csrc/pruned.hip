@@ -1,0 +1,501 @@
+// Pruned and fused 3/2-padded transforms for the Orszag products
+// (convolution.hh:495-651) on gfx950.
+//
+// The padded spectrum of an N-grid operand is nonzero only for source rows
+// (i, j in [0, N/2] U (N, 3N/2)) and kz <= N/2, and the truncation keeps
+// exactly those modes again.  So:
+//   c2r  A: axis-0 pass over the (j, kz <= N/2) source columns, reading the
+//           N-grid operand (Hessian / gradient / identity evaluated on the fly,
+//           x (3/2)^{3/2}) and writing the full padded axis;
+//        B: axis-1 pass over (i', kz <= N/2), the zero band of j' is not read;
+//        C: contiguous-axis c2r rows, kz > N/2 implicitly zero, real rows out.
+//   r2c  C': contiguous-axis r2c rows whose input is a sum of products of
+//           padded real fields (FP64 accumulation), only kz <= N/2 written;
+//        B': axis-1 pass, only the kept j' written;
+//        A': axis-0 pass, only the kept i' written straight into the N-grid
+//           with the unpad rules (Nyquist planes zeroed, / (3/2)^{3/2}), the
+//           output operator and optionally -1/k^2.
+// Padded k-space intermediates use a compact [Np][Np][N/2+1] layout; padded
+// real fields use [Np][Np][Np].  Every skipped element is an exact zero of
+// the unpruned algorithm, and every dropped output is one the unpad
+// discards, so results equal the full transforms up to rounding order.
+#include <algorithm>
+#include <cmath>
+
+#include "fft_core.hpp"
+#include "mfic_internal.hpp"
+#include "pruned.hpp"
+
+namespace mfic {
+
+// padded index -> source index (-1 in the zero band), convolution.hh:562-571
+__device__ __forceinline__ int src_of(int ip, int n) { return ip <= n / 2 ? ip : (ip > n ? ip - n / 2 : -1); }
+__device__ __forceinline__ int pad_of(int i, int n) { return i > n / 2 ? i + n / 2 : i; }
+__device__ __forceinline__ double gradc(int idx, int n, double kf) {
+    return idx == n / 2 ? 0.0 : ((double)idx - (double)(idx > n / 2) * n) * kf;
+}
+
+// ---------------------------------------------------------- loaders / stores
+
+// pass A: operand of an N-grid k-space field.  Along a pencil only the
+// axis-0 gradient g_i varies, so every operand multiplier is a polynomial
+// s(g_i) = (alpha g_i + beta) g_i + gamma fixed per column (imaginary for a
+// single gradient); the (3/2)^{3/2} pad factor is folded in.
+template <typename T> struct LdOperand {
+    const cx<T>* f;
+    int n, kc, kind, d0, d1, d2, d3;
+    double kf, scale;
+    struct Col { const cx<T>* p; double al, be, ga; bool imag; };
+    // product -(g_a g_b) as polynomial coefficients in g_i
+    __device__ __forceinline__ static void prod2(int a, int b, double gj, double gk, double sg, double& al, double& be,
+                                                 double& ga) {
+        const double ga_ = a == 1 ? gj : gk, gb_ = b == 1 ? gj : gk;
+        if (a == 0 && b == 0) al += -sg;
+        else if (a == 0) be += -sg * gb_;
+        else if (b == 0) be += -sg * ga_;
+        else ga += -sg * (ga_ * gb_);
+    }
+    __device__ __forceinline__ Col column(long col) const {
+        Col c;
+        const int j = (int)(col / kc), k = (int)(col % kc);
+        c.p = f + col;
+        const double gj = gradc(j, n, kf), gk = gradc(k, n, kf);
+        c.al = c.be = c.ga = 0.0;
+        c.imag = false;
+        switch (kind) {
+        case MFIC_OPND_HESS: prod2(d0, d1, gj, gk, 1.0, c.al, c.be, c.ga); break;
+        case MFIC_OPND_SUM_HESS:
+            prod2(d0, d1, gj, gk, 1.0, c.al, c.be, c.ga);
+            prod2(d2, d3, gj, gk, 1.0, c.al, c.be, c.ga);
+            break;
+        case MFIC_OPND_DIFF_HESS:
+            prod2(d0, d1, gj, gk, 1.0, c.al, c.be, c.ga);
+            prod2(d2, d3, gj, gk, -1.0, c.al, c.be, c.ga);
+            break;
+        case MFIC_OPND_GRAD:
+            c.imag = true;
+            if (d0 == 0) c.be = 1.0; else c.ga = d0 == 1 ? gj : gk;
+            break;
+        default: c.ga = 1.0; break;
+        }
+        c.al *= scale;
+        c.be *= scale;
+        c.ga *= scale;
+        return c;
+    }
+    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
+        const int i = src_of(pos, n);
+        if (i < 0) return cx<T>{0, 0};
+        const cx<T> v = c.p[(long)i * n * kc];
+        const double g = gradc(i, n, kf);
+        const double s = (c.al * g + c.be) * g + c.ga;
+        return c.imag ? cx<T>{(T)(-s * (double)v.y), (T)(s * (double)v.x)}
+                      : cx<T>{(T)(s * (double)v.x), (T)(s * (double)v.y)};
+    }
+};
+// pass A store: padded column (j' = pad(j)) of the compact k buffer
+template <typename T> struct StPadCol {
+    cx<T>* kb;
+    int n, np, kc;
+    struct Col { cx<T>* p; };
+    __device__ __forceinline__ Col column(long col) const {
+        const int j = (int)(col / kc), k = (int)(col % kc);
+        return Col{kb + (long)pad_of(j, n) * kc + k};
+    }
+    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const { c.p[(long)pos * np * kc] = v; }
+};
+// passes B / B': axis-1 pencils of the compact buffer, column = (i', kz)
+template <typename T> struct LdAx1 {
+    const cx<T>* kb;
+    int n, np, kc, skip_band;
+    struct Col { const cx<T>* p; };
+    __device__ __forceinline__ Col column(long col) const {
+        const long ip = col / kc, k = col % kc;
+        return Col{kb + ip * np * kc + k};
+    }
+    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
+        if (skip_band && pos > n / 2 && pos <= n) return cx<T>{0, 0};
+        return c.p[(long)pos * kc];
+    }
+};
+template <typename T> struct StAx1 {
+    cx<T>* kb;
+    int n, np, kc, drop_band;
+    struct Col { cx<T>* p; };
+    __device__ __forceinline__ Col column(long col) const {
+        const long ip = col / kc, k = col % kc;
+        return Col{kb + ip * np * kc + k};
+    }
+    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const {
+        if (drop_band && pos > n / 2 && pos <= n) return;
+        c.p[(long)pos * kc] = v;
+    }
+};
+// pass A' load: padded column of the compact buffer for kept j
+template <typename T> struct LdPadCol {
+    const cx<T>* kb;
+    int n, np, kc;
+    struct Col { const cx<T>* p; };
+    __device__ __forceinline__ Col column(long col) const {
+        const int j = (int)(col / kc), k = (int)(col % kc);
+        return Col{kb + (long)pad_of(j, n) * kc + k};
+    }
+    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const { return c.p[(long)pos * np * kc]; }
+};
+// pass A' store: unpad rules + output operator (+ optional -1/k^2) into the N-grid
+template <typename T> struct StUnpad {
+    cx<T>* out;
+    int n, kc, op, invlap;
+    double scale, coef, kf;
+    struct Col { cx<T>* p; int j, k; double kj, kk; bool nyq; };
+    __device__ __forceinline__ Col column(long col) const {
+        Col c;
+        c.j = (int)(col / kc);
+        c.k = (int)(col % kc);
+        c.p = out + col;
+        c.kj = ((double)c.j - (double)(c.j > n / 2) * n) * kf;
+        c.kk = ((double)c.k - (double)(c.k > n / 2) * n) * kf;
+        c.nyq = (c.j == n / 2) || (c.k == n / 2);
+        return c;
+    }
+    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const {
+        const int i = src_of(pos, n);
+        if (i < 0) return;
+        double re = 0.0, im = 0.0;
+        if (!(c.nyq || i == n / 2)) {
+            re = (double)v.x * scale;
+            im = (double)v.y * scale;
+            if (invlap) {
+                const double ki = ((double)i - (double)(i > n / 2) * n) * kf;
+                const double k2 = ki * ki + c.kj * c.kj + c.kk * c.kk;
+                if (k2 > 0.0) { re = -re / k2; im = -im / k2; } else { re = 0.0; im = 0.0; }
+            }
+        }
+        cx<T>* dst = c.p + (long)i * n * kc;
+        if (op == MFIC_OP_ASSIGN) { *dst = cx<T>{(T)re, (T)im}; return; }
+        const cx<T> o = *dst;
+        if (op == MFIC_OP_ADD) *dst = cx<T>{(T)((double)o.x + re), (T)((double)o.y + im)};
+        else if (op == MFIC_OP_SUB) *dst = cx<T>{(T)((double)o.x - re), (T)((double)o.y - im)};
+        else *dst = cx<T>{(T)((double)o.x + re * coef), (T)((double)o.y + im * coef)};
+    }
+};
+
+// ------------------------------------------------------------- kernels ----
+
+template <typename T, int L, int TK, bool FWD, class LD, class ST>
+__global__ void __launch_bounds__(Plan<L>::P * TK)
+k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp) {
+    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int t = threadIdx.x % TK, q = threadIdx.x / TK;
+    const long col = (long)blockIdx.x * TK + t;
+    const bool active = col < ncols;
+    const long c = active ? col : 0;
+    const auto lc = ld.column(c);
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = active ? ld.load(lc, q + P * e) : cx<T>{0, 0};
+    fft_regs<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
+    if (active) {
+        const auto sc = st.column(c);
+#pragma unroll
+        for (int e = 0; e < E; ++e) st.store(sc, q + P * e, v[e]);
+    }
+}
+
+// compact half-spectrum rows (kc <= M+1 values, the rest zero) -> real rows
+template <typename T, int M, int RW>
+__global__ void __launch_bounds__(Plan<M>::P * RW)
+k_rows_c2r_pruned(const cx<T>* __restrict__ kin, int kc, T* __restrict__ rout, long rpitch, long nrows, T scale,
+                  const cx<T>* __restrict__ tw) {
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int q = threadIdx.x % P, rw = threadIdx.x / P;
+    const long row = (long)blockIdx.x * RW + rw;
+    const bool active = row < nrows;
+    const cx<T>* src = kin + (active ? row : 0) * kc;
+    cx<T>* l = lds + rw * RP;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = q + P * e;
+        cx<T> x = (active && k < kc) ? src[k] : cx<T>{0, 0};
+        if (k == 0) x.y = 0;
+        l[k] = x;
+    }
+    if (q == 0) l[M] = (active && M < kc) ? cx<T>{src[M].x, (T)0} : cx<T>{0, 0};
+    __syncthreads();
+    const cx<T>* uw = tw + plan_tw_size<M>();
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = q + P * e;
+        const cx<T> xk = l[k], xm = l[M - k];
+        const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
+        const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
+        const cx<T> B = cmulc(D, uw[k]);
+        v[e] = cx<T>{A.x - B.y, A.y + B.x};
+    }
+    __syncthreads();
+    fft_regs<T, M, false>(v, q, lds, LdsMap{rw * RP, 1}, tw);
+    if (active) {
+        cx<T>* dst = reinterpret_cast<cx<T>*>(rout + row * rpitch);
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[q + P * e] = cscale(v[e], scale);
+    }
+}
+
+// real rows = sum_t c_t A_t B_t (FP64 sums; an array used by several terms
+// is re-read from cache) -> truncated half-spectrum rows (first kc values)
+struct RowProd {
+    const void* a[16];
+    const void* b[16];
+    double c[16];
+    int nterm, narr;
+};
+
+template <typename T, int M, int RW>
+__global__ void __launch_bounds__(Plan<M>::P * RW)
+k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long nrows, T scale,
+                const cx<T>* __restrict__ tw) {
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int q = threadIdx.x % P, rw = threadIdx.x / P;
+    const long row = (long)blockIdx.x * RW + rw;
+    const bool active = row < nrows;
+    const long off = (active ? row : 0) * rpitch;
+    cx<T> v[E];
+    // terms outermost, CH elements per chunk: all loads of a chunk are in
+    // flight together
+    constexpr int CH = (E % 8 == 0) ? 8 : (E % 6 == 0 ? 6 : (E % 4 == 0 ? 4 : E));
+#pragma unroll
+    for (int e0 = 0; e0 < E; e0 += CH) {
+        double s0[CH], s1[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) s0[u] = s1[u] = 0.0;
+        if (active)
+            for (int t = 0; t < pr.nterm; ++t) {
+                const cx<T>* A = reinterpret_cast<const cx<T>*>(static_cast<const T*>(pr.a[t]) + off);
+                const cx<T>* B = reinterpret_cast<const cx<T>*>(static_cast<const T*>(pr.b[t]) + off);
+                cx<T> x[CH], y[CH];
+#pragma unroll
+                for (int u = 0; u < CH; ++u) { x[u] = A[q + P * (e0 + u)]; y[u] = B[q + P * (e0 + u)]; }
+                const double ct = pr.c[t];
+#pragma unroll
+                for (int u = 0; u < CH; ++u) {
+                    s0[u] += ct * ((double)x[u].x * (double)y[u].x);
+                    s1[u] += ct * ((double)x[u].y * (double)y[u].y);
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) v[e0 + u] = cx<T>{(T)s0[u], (T)s1[u]};
+    }
+    fft_regs<T, M, true>(v, q, lds, LdsMap{rw * RP, 1}, tw);
+    cx<T>* l = lds + rw * RP;
+#pragma unroll
+    for (int e = 0; e < E; ++e) l[q + P * e] = v[e];
+    __syncthreads();
+    const cx<T>* uw = tw + plan_tw_size<M>();
+    if (active) {
+        cx<T>* dst = kout + row * kc;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            if (k >= kc) continue;
+            const cx<T> zk = v[e], zm = l[k == 0 ? 0 : M - k];
+            const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+            const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+            dst[k] = cscale(cadd(Ek, cmul(uw[k], Ok)), scale);
+        }
+        if (q == 0 && M < kc) {
+            const cx<T> z0 = l[0];
+            dst[M] = cx<T>{(z0.x - z0.y) * scale, (T)0};
+        }
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+
+template <typename T, int L>
+static int pick_tile(mfic_ctx ctx) {
+    constexpr int P = Plan<L>::P;
+    int tk = 64;
+    const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    if (ov != ctx->tiles.end()) tk = ov->second.first;
+    while (tk > 1 && ((size_t)L * tk * sizeof(cx<T>) > 139264 || P * tk > 1024)) tk /= 2;
+    return tk;
+}
+
+template <typename T, int L, bool FWD, class LD, class ST>
+static void launch_f(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const char* name, double bytes) {
+    constexpr int P = Plan<L>::P;
+    const int tk = pick_tile<T, L>(ctx);
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const size_t lds = (size_t)L * tk * sizeof(cx<T>);
+    dim3 grid((unsigned)((ncols + tk - 1) / tk));
+    ProfScope prof(ctx, std::string(name) + "_L" + std::to_string(L) + (sizeof(T) == 8 ? "_f64" : "_f32"), bytes);
+    switch (tk) {
+#define TKC(K)                                                                                                     \
+    case K:                                                                                                        \
+        if constexpr (P * K <= 1024)                                                                               \
+            hipLaunchKernelGGL((k_pencils_f<T, L, K, FWD, LD, ST>), grid, dim3(P * K), lds, ctx->stream, ld, st, ncols, \
+                               tw, K);                                                                             \
+        break;
+        TKC(1) TKC(2) TKC(4) TKC(8) TKC(16) TKC(32) TKC(64)
+#undef TKC
+    default:
+        throw Error(MFIC_ERR_SIZE, "bad tile");
+    }
+    MFIC_LAUNCH_CHECK();
+}
+
+template <typename T, int M>
+static int pick_rows(mfic_ctx ctx, bool r2c) {
+    constexpr int P = Plan<M>::P;
+    int rw = std::max(1, std::min(64, (r2c ? 512 : 256) / P / (M >= 256 ? 4 : 1)));
+    int p2 = 1;
+    while (p2 * 2 <= rw) p2 *= 2;
+    rw = p2;
+    const auto ov = ctx->tiles.find(std::make_tuple(M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    if (ov != ctx->tiles.end()) rw = ov->second.first;
+    while (rw > 1 && ((size_t)(M + 1) * rw * sizeof(cx<T>) > 139264 || P * rw > 1024)) rw /= 2;
+    return rw;
+}
+
+template <typename T, int M>
+static void rows_c2r_M(mfic_ctx ctx, const cx<T>* kin, int kc, T* rout, long rpitch, long nrows, T scale) {
+    constexpr int P = Plan<M>::P;
+    const int rw = pick_rows<T, M>(ctx, false);
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
+    dim3 grid((unsigned)((nrows + rw - 1) / rw));
+    ProfScope prof(ctx, std::string("fft_c2rC_n") + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   (double)nrows * ((double)kc * sizeof(cx<T>) + 2.0 * M * sizeof(T)));
+    switch (rw) {
+#define RWC(K)                                                                                                   \
+    case K:                                                                                                      \
+        if constexpr (P * K <= 1024)                                                                             \
+            hipLaunchKernelGGL((k_rows_c2r_pruned<T, M, K>), grid, dim3(P * K), lds, ctx->stream, kin, kc, rout, \
+                               rpitch, nrows, scale, tw);                                                        \
+        break;
+        RWC(1) RWC(2) RWC(4) RWC(8) RWC(16) RWC(32) RWC(64)
+#undef RWC
+    default:
+        throw Error(MFIC_ERR_SIZE, "bad row tile");
+    }
+    MFIC_LAUNCH_CHECK();
+}
+
+template <typename T, int M>
+static void rows_r2c_M(mfic_ctx ctx, const RowProd& pr, long rpitch, cx<T>* kout, int kc, long nrows, T scale) {
+    constexpr int P = Plan<M>::P;
+    const int rw = pick_rows<T, M>(ctx, true);
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
+    dim3 grid((unsigned)((nrows + rw - 1) / rw));
+    ProfScope prof(ctx, std::string("fft_r2cC_n") + std::to_string(2 * M) + "_a" + std::to_string(pr.narr) +
+                            (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   (double)nrows * (2.0 * M * sizeof(T) * pr.narr + (double)kc * sizeof(cx<T>)));
+    switch (rw) {
+#define RWC(K)                                                                                                 \
+    case K:                                                                                                    \
+        if constexpr (P * K <= 1024)                                                                           \
+            hipLaunchKernelGGL((k_rows_r2c_prod<T, M, K>), grid, dim3(P * K), lds, ctx->stream, pr, rpitch, kout, \
+                               kc, nrows, scale, tw);                                                          \
+        break;
+        RWC(1) RWC(2) RWC(4) RWC(8) RWC(16) RWC(32) RWC(64)
+#undef RWC
+    default:
+        throw Error(MFIC_ERR_SIZE, "bad row tile");
+    }
+    MFIC_LAUNCH_CHECK();
+}
+
+#define MFIC_PADDED_SIZES(X) X(12) X(18) X(24) X(36) X(48) X(72) X(96) X(144) X(192) X(288) X(384) X(576) X(768)
+
+template <typename T>
+static void c2r_from_operand_t(mfic_ctx ctx, const OperandDesc& o, size_t n, double L, void* kbuf, void* rout) {
+    const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
+    const double rfac = std::pow(1.5, 1.5);
+    const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
+    const double kf = 2.0 * 3.14159265358979323846 / L;
+    LdOperand<T> la{static_cast<const cx<T>*>(o.f), N, kc, o.kind, o.d[0], o.d[1], o.d[2], o.d[3], kf, rfac};
+    StPadCol<T> sa{static_cast<cx<T>*>(kbuf), N, np, kc};
+    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 1};
+    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 0};
+    const double cs = sizeof(cx<T>);
+    const long colsA = (long)N * kc, colsB = (long)np * kc;
+    switch (np) {
+#define X(S)                                                                                                 \
+    case S:                                                                                                  \
+        launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs);                    \
+        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs);                    \
+        rows_c2r_M<T, S / 2>(ctx, static_cast<const cx<T>*>(kbuf), kc, static_cast<T*>(rout), np, (long)np * np, \
+                             (T)norm);                                                                       \
+        break;
+        MFIC_PADDED_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
+    }
+}
+
+template <typename T>
+static void r2c_products_t(mfic_ctx ctx, const std::vector<ProdTerm>& terms, size_t n, double L, void* kbuf, void* out,
+                           int op, double coef, bool invlap) {
+    const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
+    RowProd pr{};
+    std::vector<const void*> arrs;
+    MFIC_CHECK(!terms.empty() && terms.size() <= 16, MFIC_ERR_ARG, "1..16 product terms");
+    auto slot = [&](const void* p) {
+        auto it = std::find(arrs.begin(), arrs.end(), p);
+        if (it != arrs.end()) return (int)(it - arrs.begin());
+        arrs.push_back(p);
+        return (int)arrs.size() - 1;
+    };
+    for (size_t t = 0; t < terms.size(); ++t) {
+        slot(terms[t].a);
+        slot(terms[t].b);
+        pr.a[t] = terms[t].a;
+        pr.b[t] = terms[t].b;
+        pr.c[t] = terms[t].c;
+    }
+    pr.narr = (int)arrs.size();
+    pr.nterm = (int)terms.size();
+    const double rfac = std::sqrt((double)np * np * np) / std::sqrt((double)N * N * N);
+    const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
+    const double kf = 2.0 * 3.14159265358979323846 / L;
+    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 0};
+    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 1};
+    LdPadCol<T> la{static_cast<const cx<T>*>(kbuf), N, np, kc};
+    StUnpad<T> sa{static_cast<cx<T>*>(out), N, kc, op, invlap ? 1 : 0, norm / rfac, coef, kf};
+    const double cs = sizeof(cx<T>);
+    const long colsA = (long)N * kc, colsB = (long)np * kc;
+    switch (np) {
+#define X(S)                                                                                                        \
+    case S:                                                                                                         \
+        rows_r2c_M<T, S / 2>(ctx, pr, np, static_cast<cx<T>*>(kbuf), kc, (long)np * np, (T)1);                      \
+        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs);                            \
+        launch_f<T, S, true>(ctx, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
+        break;
+        MFIC_PADDED_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
+    }
+}
+
+void padded_c2r_from_operand(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf, void* rout) {
+    if (prec == MFIC_F64) c2r_from_operand_t<double>(ctx, o, n, L, kbuf, rout);
+    else c2r_from_operand_t<float>(ctx, o, n, L, kbuf, rout);
+}
+
+void padded_r2c_products(mfic_ctx ctx, int prec, const std::vector<ProdTerm>& terms, size_t n, double L, void* kbuf,
+                         void* out, int op, double coef, bool invlap) {
+    if (prec == MFIC_F64) r2c_products_t<double>(ctx, terms, n, L, kbuf, out, op, coef, invlap);
+    else r2c_products_t<float>(ctx, terms, n, L, kbuf, out, op, coef, invlap);
+}
+
+}  // namespace mfic

@@ -1,0 +1,24 @@
+// Pruned / fused 3/2-padded transforms (pruned.hip).
+#pragma once
+#include <vector>
+
+#include "mfic_internal.hpp"
+#include "orszag.hpp"
+
+namespace mfic {
+
+struct ProdTerm {
+    const void* a;  // padded real field [Np][Np][Np]
+    const void* b;
+    double c;
+};
+
+// operand of an N-grid k-space field -> padded real field rout ([Np]^3),
+// through the compact padded k buffer kbuf ([Np][Np][N/2+1] complex)
+void padded_c2r_from_operand(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf, void* rout);
+// sum_t c_t a_t b_t (padded real) -> truncated N-grid k-space with the unpad
+// rules, output operator op / coef and optional -1/k^2 (DC := 0)
+void padded_r2c_products(mfic_ctx ctx, int prec, const std::vector<ProdTerm>& terms, size_t n, double L, void* kbuf,
+                         void* out, int op, double coef, bool invlap);
+
+}  // namespace mfic
