@@ -218,7 +218,15 @@ void plt_build(mfic_ctx ctx, int lat, int ng, PltTables& out) {
                 auto re = [&](int c) { return hD[c * nr + 2 * q] / fft_norm12; };
                 const double A[9] = {re(0), re(1), re(2), re(1), re(3), re(4), re(2), re(4), re(5)};
                 double ev[3], V[9];
-                jacobi3(A, ev, V);
+                if (q == 0) {
+                    // k = 0: D is isotropic in exact arithmetic and any basis
+                    // diagonalises it; use the Cartesian one (largest-
+                    // eigenvalue vector e_z) instead of rounding noise
+                    for (int c = 0; c < 9; ++c) V[c] = (c % 4 == 0) ? 1.0 : 0.0;
+                    ev[0] = ev[1] = ev[2] = (A[0] + A[4] + A[8]) / 3.0;
+                } else {
+                    jacobi3(A, ev, V);
+                }
                 const double s = 2.0 * kPi * ng;
                 const double e3[3] = {V[2] / s, V[5] / s, V[8] / s};
                 const double arv[3] = {sidx(i) * 2.0 * kPi / s, sidx(j) * 2.0 * kPi / s, sidx(k) * 2.0 * kPi / s};

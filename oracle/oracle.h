@@ -182,6 +182,9 @@ typedef struct orc_plt {
     double *Dr, *Dtheta, *Dphi, *vfac;   /* ng x ng x (ng/2+1) real parts */
 } orc_plt;
 int orc_plt_init(orc_plt *p, int lattice, int ng);
+/* the normalised k-space dynamical matrix D_xx D_xy D_xz D_yy D_yz D_zz
+ * (real parts / ng^-3/2), six ng x ng x (ng/2+1) arrays, before the eigen step */
+void orc_plt_dynmat(int lattice, int ng, double *out6);
 void orc_plt_free(orc_plt *p);
 /* gradient(idim, ijk) imaginary part and vfac_corr(ijk) for an N-grid */
 double orc_plt_gradient(const orc_plt *p, int idim, int n, double L, int i, int j, int k);

@@ -127,6 +127,7 @@ def lib():
         L.orc_plt_init.argtypes = [C.POINTER(Plt), C.c_int, C.c_int]
         L.orc_plt_init.restype = C.c_int
         L.orc_plt_free.argtypes = [C.POINTER(Plt)]
+        L.orc_plt_dynmat.argtypes = [C.c_int, C.c_int, _dp]
         L.orc_plt_gradient.restype = C.c_double
         L.orc_plt_gradient.argtypes = [C.POINTER(Plt), C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]
         L.orc_plt_vfac_corr.restype = C.c_double
@@ -299,6 +300,13 @@ def plt_tables(lattice: int, ng: int = 64) -> Plt:
     if lib().orc_plt_init(C.byref(p), lattice, ng) != 0:
         raise RuntimeError("PLT init failed")
     return p
+
+
+def plt_dynmat(lattice: int, ng: int) -> np.ndarray:
+    """(6, ng, ng, ng/2+1): D_xx D_xy D_xz D_yy D_yz D_zz in k-space"""
+    out = np.zeros(6 * ng * ng * (ng // 2 + 1))
+    lib().orc_plt_dynmat(lattice, ng, out)
+    return out.reshape(6, ng, ng, ng // 2 + 1)
 
 
 def plt_arrays(p: Plt):

@@ -126,19 +126,17 @@ static inline int key_of(int x, int y, int z) {
     return ((x + KEYR) * (2 * KEYR + 1) + (y + KEYR)) * (2 * KEYR + 1) + (z + KEYR);
 }
 
-int orc_plt_init(orc_plt *P, int lat, int ng) {
-    const int nc = ng / 2 + 1;
-    const size_t nr = (size_t)ng * ng * (ng + 2), nk = (size_t)ng * ng * nc;
+/* the six real-space dynamical-matrix components on the site mesh, unitary
+ * r2c transformed in place (particle_plt.hh:160-315) */
+static void plt_dynmat(int lat, int ng, double *D[6]) {
     const double dx = 1.0 / ng;
     const double eta = 4.0, alpha = 1.0 / sqrt(2) / eta, alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
     const double charge = 1.0 / pow((double)ng, 3) / CHARGE_FAC[lat];
-    const double fft_norm = 1.0 / pow((double)ng, 3.0), fft_norm12 = 1.0 / pow((double)ng, 1.5);
+    const double fft_norm = 1.0 / pow((double)ng, 3.0);
     const double sqrtpi = sqrt(PI_), pi32 = pow(PI_, 1.5);
     double Rm[9];
     reciprocal(lat, Rm);
     const double *B = BRAV[lat];
-    double *D[6];
-    for (int c = 0; c < 6; ++c) D[c] = (double *)calloc(nr, sizeof(double));
 
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < ng; ++i)
@@ -188,6 +186,30 @@ int orc_plt_init(orc_plt *P, int lat, int ng) {
     D[0][0] = 1.0 / 3.0; D[1][0] = 0.0; D[2][0] = 0.0;
     D[3][0] = 1.0 / 3.0; D[4][0] = 0.0; D[5][0] = 1.0 / 3.0;
     for (int c = 0; c < 6; ++c) orc_fft3d_r2c(D[c], ng, ng, ng);
+}
+
+void orc_plt_dynmat(int lat, int ng, double *out6) {
+    const int nc = ng / 2 + 1;
+    const size_t nr = (size_t)ng * ng * (ng + 2), nk = (size_t)ng * ng * nc;
+    const double fft_norm12 = 1.0 / pow((double)ng, 1.5);
+    double *D[6];
+    for (int c = 0; c < 6; ++c) D[c] = (double *)calloc(nr, sizeof(double));
+    plt_dynmat(lat, ng, D);
+    for (int c = 0; c < 6; ++c) {
+        for (size_t q = 0; q < nk; ++q) out6[c * nk + q] = D[c][2 * q] / fft_norm12;
+        free(D[c]);
+    }
+}
+
+int orc_plt_init(orc_plt *P, int lat, int ng) {
+    const int nc = ng / 2 + 1;
+    const size_t nr = (size_t)ng * ng * (ng + 2), nk = (size_t)ng * ng * nc;
+    const double fft_norm12 = 1.0 / pow((double)ng, 1.5);
+    double Rm[9];
+    reciprocal(lat, Rm);
+    double *D[6];
+    for (int c = 0; c < 6; ++c) D[c] = (double *)calloc(nr, sizeof(double));
+    plt_dynmat(lat, ng, D);
 
     /* per mode: eigen data, default = packed D values (never read unless set) */
     double *t6 = (double *)malloc(sizeof(double) * 6 * nk);   /* l0 l1 l2 v0 v1 v2 */
@@ -210,7 +232,17 @@ int orc_plt_init(orc_plt *P, int lat, int ng) {
                 A[5] = A[7] = D[4][2 * q] / fft_norm12;
                 A[8] = D[5][2 * q] / fft_norm12;
                 double ev[3], V[9];
-                eigen3(A, ev, V);
+                if (q == 0) {
+                    /* k = 0: D is isotropic in exact arithmetic, so every
+                     * vector is an eigenvector and a solver's choice follows
+                     * rounding noise; take the Cartesian basis, which a
+                     * symmetric eigensolver returns for an exactly isotropic
+                     * matrix (largest-eigenvalue vector = e_z) */
+                    for (int c = 0; c < 9; ++c) V[c] = (c % 4 == 0) ? 1.0 : 0.0;
+                    ev[0] = ev[1] = ev[2] = (A[0] + A[4] + A[8]) / 3.0;
+                } else {
+                    eigen3(A, ev, V);
+                }
                 double e3[3] = {V[2] / (2.0 * PI_ * ng), V[5] / (2.0 * PI_ * ng), V[8] / (2.0 * PI_ * ng)};
                 double arv[3] = {kv[0] / (2.0 * PI_ * ng), kv[1] / (2.0 * PI_ * ng), kv[2] / (2.0 * PI_ * ng)}, a[3];
                 mv(Rm, arv, a);
