@@ -13,6 +13,8 @@ CPU FP64 oracle on a bounded 128^3 sample of the same configuration),
 fields, rel-L2 / max error per component).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
+       python bench.py --calibrate-copy          (copy kernel for counter scaling)
+       python bench.py --summarize-pmc DIR       (counter table from rocprofv3 runs)
 """
 import argparse
 import ctypes as C
@@ -79,11 +81,48 @@ def roofline(rep, steps):
     fft_bytes = sum(x["bytes_total"] for x in fft.values())
     return {
         "kernel": name, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
         "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per, "launches_per_step": v["count"] / steps,
         "all_fft_passes_GBs": round(fft_bytes / (fft_tot * 1e-3) / 1e9, 1),
         "fft_time_fraction": round(fft_tot / tot, 3),
     }
+
+
+# HIP kernel symbol (prefix) behind each profiled pass, for the rocprofv3
+# counter summary lookup
+SYMBOLS = [
+    ("fft_c2rC_n", "k_rows_c2r_pruned<"),
+    ("fft_r2cC_n", "k_rows_r2c_prod<"),
+    ("fft_c2rA_", "LdOperand<"),
+    ("fft_c2rB_", "false, mfic::LdAx1<"),
+    ("fft_r2cB_", "true, mfic::LdAx1<"),
+    ("fft_r2cA_", "LdPadCol<"),
+    ("fft_rows_c2r_", "k_rows_c2r<"),
+    ("fft_rows_r2c_", "k_rows_r2c<"),
+    ("readout3", "k_readout3<"),
+    ("assemble_all", "k_assemble_all<"),
+]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_3lpt_plt_bcc_512_f32.txt")
+
+
+def measured_traffic(name):
+    """HBM bytes per dispatch of the kernel behind pass `name` from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or
+    None when the summary does not cover it"""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    prec = "double" if name.endswith("_f64") else "float"
+    sym = next((s for p, s in SYMBOLS if name.startswith(p)), None)
+    if sym is None or name.startswith("fft_r2cC_n"):
+        return None  # product-row launches of every width share one kernel symbol
+    hits = []
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("#") or not line.strip():
+            continue
+        kname, _, fetch, write = line.rstrip("\n").split("\t")[:4]
+        if sym in kname and f"<{prec}" in kname:
+            hits.append(float(fetch) + float(write))
+    return round(sum(hits) / len(hits)) if len(hits) == 1 else None
 
 
 def kernel_table(rep, steps):
@@ -134,7 +173,70 @@ def accuracy(ctx, n, box):
     return res
 
 
+def calibrate_copy(n=512, reps=10):
+    """plain device-to-device copies of a known size (n^3 FP32 grid): under
+    rocprofv3 --pmc they give the FETCH_SIZE / WRITE_SIZE scale factors"""
+    from monofonic_amd import F32, Context, Grid
+    ctx = Context(0)
+    a = Grid(ctx, (n,) * 3, (1.0,) * 3, prec=F32)
+    b = Grid(ctx, (n,) * 3, (1.0,) * 3, prec=F32)
+    a.zero()
+    for _ in range(reps):
+        b.copy_from(a)
+    ctx.sync()
+    nbytes = n * n * (n + 2) * 4
+    print(json.dumps({"copy_bytes": nbytes, "copies": reps}), flush=True)
+    a.close()
+    b.close()
+    ctx.close()
+
+
+def _pmc_rows(d):
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    return list(csv.DictReader(open(files[0])))
+
+
+def summarize_pmc(d):
+    """d holds rocprofv3 --pmc runs in fetch/, write/ (the workload) and
+    copy_fetch/, copy_write/ (calibrate_copy); prints the per-kernel table
+    (KiB counters scaled to bytes by the copy kernel's known size)"""
+    import collections
+    nbytes = 512 * 512 * 514 * 4
+
+    def per_kernel(rows, counter):
+        acc, cnt = collections.defaultdict(float), collections.Counter()
+        for r in rows:
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"]] += float(r["Counter_Value"])
+                cnt[r["Kernel_Name"]] += 1
+        return {k: (acc[k] / cnt[k], cnt[k]) for k in acc}
+
+    def copy_factor(sub, counter):
+        pk = per_kernel(_pmc_rows(os.path.join(d, sub)), counter)
+        big = [v for k, (v, c) in pk.items() if "copyBuffer" in k and v * 1024 > 0.25 * nbytes]
+        return nbytes / (sum(big) / len(big) * 1024)
+
+    ff, fw = copy_factor("copy_fetch", "FETCH_SIZE"), copy_factor("copy_write", "WRITE_SIZE")
+    fetch = per_kernel(_pmc_rows(os.path.join(d, "fetch")), "FETCH_SIZE")
+    write = per_kernel(_pmc_rows(os.path.join(d, "write")), "WRITE_SIZE")
+    print(f"# HBM traffic per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB), scaled by a plain copy of "
+          f"{nbytes} bytes: FETCH x {ff:.4f}, WRITE x {fw:.4f}")
+    print("# kernel\tdispatches\tfetch_bytes\twrite_bytes")
+    for k in sorted(fetch, key=lambda k: -fetch[k][0] * fetch[k][1]):
+        f, c = fetch[k]
+        w = write.get(k, (0.0, 0))[0]
+        print(f"{k}\t{c}\t{f * 1024 * ff:.0f}\t{w * 1024 * fw:.0f}")
+
+
 def main():
+    if "--calibrate-copy" in sys.argv:
+        return calibrate_copy()
+    if "--summarize-pmc" in sys.argv:
+        return summarize_pmc(sys.argv[sys.argv.index("--summarize-pmc") + 1])
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)

@@ -44,33 +44,34 @@ k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
 template <typename T, int M, int RW>
 __global__ void __launch_bounds__(Plan<M>::P * RW)
 k_rows_r2c(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* __restrict__ tw) {
-    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = row_lds_pitch<M>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int q = threadIdx.x % P, rw = threadIdx.x / P;
+    const LdsRowMap lm{rw * RP};
     const long row = (long)blockIdx.x * RW + rw;
     const bool active = row < nrows;
     cx<T>* r = reinterpret_cast<cx<T>*>(data + (active ? row : 0) * pitch);
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? r[q + P * e] : cx<T>{0, 0};
-    fft_regs<T, M, true>(v, q, lds, LdsMap{rw * RP, 1}, tw);
+    fft_regs<T, M, true>(v, q, lds, LdsRowMap{rw * RP}, tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) lds[rw * RP + q + P * e] = v[e];
+    for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
     __syncthreads();
     const cx<T>* uw = tw + plan_tw_size<M>();
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int k = q + P * e;
-            const cx<T> zk = v[e], zm = lds[rw * RP + (k == 0 ? 0 : M - k)];
+            const cx<T> zk = v[e], zm = lds[lm.at(k == 0 ? 0 : M - k)];
             const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
             const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
             const cx<T> X = cadd(Ek, cmul(uw[k], Ok));
             r[k] = cscale(X, scale);
         }
         if (q == 0) {
-            const cx<T> z0 = lds[rw * RP];
+            const cx<T> z0 = lds[lm.at(0)];
             r[M] = cx<T>{(z0.x - z0.y) * scale, (T)0};
         }
     }
@@ -81,10 +82,11 @@ k_rows_r2c(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* _
 template <typename T, int M, int RW>
 __global__ void __launch_bounds__(Plan<M>::P * RW)
 k_rows_c2r(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* __restrict__ tw) {
-    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = row_lds_pitch<M>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int q = threadIdx.x % P, rw = threadIdx.x / P;
+    const LdsRowMap lm{rw * RP};
     const long row = (long)blockIdx.x * RW + rw;
     const bool active = row < nrows;
     cx<T>* r = reinterpret_cast<cx<T>*>(data + (active ? row : 0) * pitch);
@@ -93,16 +95,16 @@ k_rows_c2r(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* _
     for (int e = 0; e < E; ++e) {
         cx<T> x = active ? r[q + P * e] : cx<T>{0, 0};
         if (q + P * e == 0) x.y = 0;
-        lds[rw * RP + q + P * e] = x;
+        lds[lm.at(q + P * e)] = x;
         v[e] = x;
     }
-    if (q == 0) lds[rw * RP + M] = active ? cx<T>{r[M].x, (T)0} : cx<T>{0, 0};
+    if (q == 0) lds[lm.at(M)] = active ? cx<T>{r[M].x, (T)0} : cx<T>{0, 0};
     __syncthreads();
     const cx<T>* uw = tw + plan_tw_size<M>();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int k = q + P * e;
-        const cx<T> xk = v[e], xm = lds[rw * RP + M - k];
+        const cx<T> xk = v[e], xm = lds[lm.at(M - k)];
         // A = X_k + conj(X_{M-k}),  B = (X_k - conj(X_{M-k})) conj(W^k)
         const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
         const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
@@ -110,7 +112,7 @@ k_rows_c2r(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* _
         v[e] = cx<T>{A.x - B.y, A.y + B.x};  // A + i B
     }
     __syncthreads();
-    fft_regs<T, M, false>(v, q, lds, LdsMap{rw * RP, 1}, tw);
+    fft_regs<T, M, false>(v, q, lds, LdsRowMap{rw * RP}, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) r[q + P * e] = cscale(v[e], scale);
@@ -267,10 +269,10 @@ static void launch_rows_M(mfic_ctx ctx, T* data, size_t nrows, long pitch, T sca
     int rw = std::max(1, std::min(64, pow2_floor((R2C ? 512 : 256) / P / (M >= 256 ? 4 : 1))));
     const auto ov = ctx->tiles.find(std::make_tuple(M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) rw = ov->second.first;
-    while (rw > 1 && (size_t)(M + 1) * rw * sizeof(cx<T>) > max_lds(ctx)) rw /= 2;
+    while (rw > 1 && (size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>) > max_lds(ctx)) rw /= 2;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
-    size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
+    size_t lds = (size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>);
     ProfScope prof(ctx, std::string(R2C ? "fft_rows_r2c_n" : "fft_rows_c2r_n") + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    (double)nrows * (2.0 * M + 2.0 * (M + 1)) * sizeof(T));
     switch (rw) {

@@ -181,9 +181,34 @@ struct LdsMap {
     __device__ __forceinline__ int at(int pos) const { return base + pos * stride; }
 };
 
+// LDS addressing for a transform whose threads all work on one row: one pad
+// slot every 16 elements spreads the stride-R butterfly writes of the early
+// Stockham stages over the LDS banks
+struct LdsRowMap {
+    int base;
+    __device__ __forceinline__ int at(int pos) const { return base + pos + (pos >> 4); }
+};
+// elements one row of M + 1 complex values occupies under LdsRowMap
+template <int M> constexpr int row_lds_pitch() { return M + 1 + ((M + 1) >> 4); }
+
+// LDS exchange synchronisation between Stockham stages
+struct BlockSync {
+    __device__ static __forceinline__ void sync() { __syncthreads(); }
+};
+// for a transform whose threads all sit in one wavefront: a wavefront's LDS
+// operations are performed in issue order, so it suffices to keep the
+// compiler from moving memory operations across this point
+struct WaveSync {
+    __device__ static __forceinline__ void sync() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
 // one Stockham stage on registers v[E]; exchanges through LDS unless last
-template <typename T, int L, int S, bool FWD>
-__device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, LdsMap m, const cx<T>* __restrict__ tw) {
+template <typename T, int L, int S, bool FWD, class MAP, class SYNC = BlockSync>
+__device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, const cx<T>* __restrict__ tw) {
     using PL = Plan<L>;
     constexpr int E = PL::E, P = PL::P, R = PL::r(S), B = E / R, NS = plan_ns<L>(S);
     constexpr bool LAST = (S == PL::S - 1);
@@ -213,21 +238,21 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, LdsMap m,
         }
     }
     if (!LAST) {
-        __syncthreads();
+        SYNC::sync();
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = lds[m.at(q + P * e)];
-        __syncthreads();
+        SYNC::sync();
     }
 }
 
 // full transform of length L on registers v[E] (positions q + P e)
-template <typename T, int L, bool FWD>
-__device__ __forceinline__ void fft_regs(cx<T>* v, int q, cx<T>* lds, LdsMap m, const cx<T>* __restrict__ tw) {
+template <typename T, int L, bool FWD, class MAP, class SYNC = BlockSync>
+__device__ __forceinline__ void fft_regs(cx<T>* v, int q, cx<T>* lds, MAP m, const cx<T>* __restrict__ tw) {
     constexpr int S = Plan<L>::S;
-    fft_stage<T, L, 0, FWD>(v, q, lds, m, tw);
-    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD>(v, q, lds, m, tw);
-    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD>(v, q, lds, m, tw);
-    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD>(v, q, lds, m, tw);
+    fft_stage<T, L, 0, FWD, MAP, SYNC>(v, q, lds, m, tw);
+    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
+    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
+    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
 }
 
 }  // namespace mfic

@@ -1,0 +1,235 @@
+// Fused padded row pass for the LPT sources (ic_generator.cc:523-629 with
+// the Orszag 3/2 rule of convolution.hh:495-532).
+//
+// One workgroup takes one or a few padded rows (i', j').  Each input field's
+// compact half-spectrum row (kz <= N/2, after the axis-0 and axis-1 c2r
+// passes) is brought to real space in LDS by its own thread group, the
+// source products are formed pointwise from LDS with FP64 sums, and every
+// output row is transformed back (r2c) and written as a compact
+// half-spectrum row for the axis-1 / axis-0 r2c passes.  The padded real
+// fields therefore never reach HBM: per row the pass reads NI and writes NO
+// compact rows.
+//
+// Each transform's threads lie inside one wavefront (groups start on
+// power-of-two lane boundaries), so Stockham stages synchronise per
+// wavefront and idle output groups simply skip the r2c; workgroup barriers
+// separate the three phases.
+#include <cmath>
+
+#include "fft_core.hpp"
+#include "fused.hpp"
+#include "mfic_internal.hpp"
+
+namespace mfic {
+
+// Hessian component order in every source: xx yy zz xy xz yz
+// 2LPT source (ic_generator.cc:536-541)
+struct ProdSrc2 {
+    static constexpr int NI = 6, NO = 1;
+    __device__ static __forceinline__ void eval(const double* h, double* o) {
+        o[0] = h[0] * (h[1] + h[2]) + h[1] * h[2] - h[3] * h[3] - h[4] * h[4] - h[5] * h[5];
+    }
+};
+// 2LPT source plus the inner products of the 3a terms grouped by their outer
+// factor: Qzz = xx yy - xy^2, Qyz = xy xz, Qxx = yz^2, Qyy = xz^2
+// (ic_generator.cc:581-585)
+struct ProdSrc2Q {
+    static constexpr int NI = 6, NO = 5;
+    __device__ static __forceinline__ void eval(const double* h, double* o) {
+        ProdSrc2::eval(h, o);
+        o[1] = h[0] * h[1] - h[3] * h[3];
+        o[2] = h[3] * h[4];
+        o[3] = h[5] * h[5];
+        o[4] = h[4] * h[4];
+    }
+};
+// phi3 source (3a outer products with the truncated Q fields + 3b,
+// ic_generator.cc:581-609) and A3 (ic_generator.cc:614-627).  Inputs: phi1
+// Hessians a[0..5], phi2 Hessians b[6..11], Qzz Qyz Qxx Qyy [12..15]
+struct ProdSrc3 {
+    static constexpr int NI = 16, NO = 4;
+    __device__ static __forceinline__ void eval(const double* h, double* o) {
+        const double* a = h;
+        const double* b = h + 6;
+        const double* Q = h + 12;
+        const double c5 = -5.0 / 7.0, c10 = 10.0 / 7.0;
+        o[0] = Q[0] * a[2] + 2.0 * (Q[1] * a[5]) - Q[2] * a[0] - Q[3] * a[1] +
+               c5 * (a[0] * (b[1] + b[2]) + a[1] * (b[2] + b[0]) + a[2] * (b[0] + b[1])) +
+               c10 * (a[3] * b[3] + a[4] * b[4] + a[5] * b[5]);
+        // A3[d] = b_{dd'} a_{dd''} - b_{dd''} a_{dd'} + a_{d'd''} (b_{d'd'} - b_{d''d''})
+        //         - b_{d'd''} (a_{d'd'} - a_{d''d''}),  d' = d+1, d'' = d+2 (mod 3)
+        o[1] = b[3] * a[4] - b[4] * a[3] + a[5] * (b[1] - b[2]) - b[5] * (a[1] - a[2]);
+        o[2] = b[5] * a[3] - b[3] * a[5] + a[4] * (b[2] - b[0]) - b[4] * (a[2] - a[0]);
+        o[3] = b[4] * a[5] - b[5] * a[4] + a[3] * (b[0] - b[1]) - b[3] * (a[0] - a[1]);
+    }
+};
+
+template <int P> constexpr int lane_slot() { return P <= 1 ? 1 : P <= 2 ? 2 : P <= 4 ? 4 : P <= 8 ? 8 : P <= 16 ? 16 : P <= 32 ? 32 : 64; }
+template <int M, int NI> constexpr int fused_rows_per_block() {
+    constexpr int tpr = NI * lane_slot<Plan<M>::P>();
+    return tpr >= 256 ? 1 : (256 / tpr > 8 ? 8 : 256 / tpr);
+}
+
+template <typename T, int M, class PR>
+__global__ void __launch_bounds__((fused_rows_per_block<M, PR::NI>() * PR::NI * lane_slot<Plan<M>::P>()))
+k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out, long ostride, int kc, long nrows,
+             T scale, const cx<T>* __restrict__ tw) {
+    constexpr int NI = PR::NI, NO = PR::NO, E = Plan<M>::E, P = Plan<M>::P, SL = lane_slot<P>();
+    constexpr int RP = row_lds_pitch<M>(), RB = fused_rows_per_block<M, NI>(), TPR = NI * SL;
+    constexpr int NPOS = (M + TPR - 1) / TPR;
+    static_assert(P <= 64 && 64 % SL == 0, "a transform must fit in one wavefront");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int tr = threadIdx.x % TPR, rb = threadIdx.x / TPR;
+    const int g = tr / SL, q = tr % SL;
+    const bool lane = q < P;
+    const long row = (long)blockIdx.x * RB + rb;
+    const bool active = row < nrows;
+    const LdsRowMap lm{(rb * NI + g) * RP};
+    const cx<T>* uw = tw + plan_tw_size<M>();
+
+    // 1. c2r of input g: half-spectrum row (kz < kc, kz = 0 and the padded
+    // Nyquist real) -> M complex = 2M real values, kept in the group's region
+    if (lane) {
+        const cx<T>* src = in + g * istride + (active ? row : 0) * (long)kc;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            cx<T> x = (active && k < kc) ? src[k] : cx<T>{0, 0};
+            if (k == 0) x.y = 0;
+            lds[lm.at(k)] = x;
+        }
+        if (q == 0) lds[lm.at(M)] = cx<T>{0, 0};
+        WaveSync::sync();
+        cx<T> v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            const cx<T> xk = lds[lm.at(k)], xm = lds[lm.at(M - k)];
+            const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
+            const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
+            const cx<T> B = cmulc(D, uw[k]);
+            v[e] = cx<T>{A.x - B.y, A.y + B.x};
+        }
+        WaveSync::sync();
+        fft_regs<T, M, false, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = cscale(v[e], scale);
+    }
+    __syncthreads();
+
+    // 2. source products at each complex slot (two real points), FP64 sums
+    // rounded once to T
+    T acc[NPOS][NO][2];
+#pragma unroll
+    for (int u = 0; u < NPOS; ++u) {
+        const int p = tr + u * TPR;
+        if (p < M) {
+            cx<T> x[NI];
+#pragma unroll
+            for (int f = 0; f < NI; ++f) x[f] = lds[(rb * NI + f) * RP + p + (p >> 4)];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double h[NI], o[NO];
+#pragma unroll
+                for (int f = 0; f < NI; ++f) h[f] = (double)(c ? x[f].y : x[f].x);
+                PR::eval(h, o);
+#pragma unroll
+                for (int r = 0; r < NO; ++r) acc[u][r][c] = (T)o[r];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NPOS; ++u) {
+        const int p = tr + u * TPR;
+        if (p < M) {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) lds[(rb * NI + o) * RP + p + (p >> 4)] = cx<T>{acc[u][o][0], acc[u][o][1]};
+        }
+    }
+    __syncthreads();
+
+    // 3. r2c of output g < NO, first kc half-spectrum values written
+    if (lane && g < NO) {
+        cx<T> v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = lds[lm.at(q + P * e)];
+        WaveSync::sync();
+        fft_regs<T, M, true, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
+        WaveSync::sync();
+        if (active) {
+            cx<T>* dst = out + g * ostride + row * (long)kc;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = q + P * e;
+                if (k >= kc) continue;
+                const cx<T> zk = v[e], zm = lds[lm.at(k == 0 ? 0 : M - k)];
+                const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+                const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+                dst[k] = cadd(Ek, cmul(uw[k], Ok));
+            }
+        }
+    }
+}
+
+template <typename T, int M, class PR>
+static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, int kc, long nrows, double scale,
+                         const char* name) {
+    constexpr int NI = PR::NI, NO = PR::NO, SL = lane_slot<Plan<M>::P>();
+    constexpr int RB = fused_rows_per_block<M, NI>(), RP = row_lds_pitch<M>();
+    MFIC_CHECK(kc <= M, MFIC_ERR_SIZE, "half-spectrum rows longer than the padded transform");
+    const size_t lds = (size_t)RB * NI * RP * sizeof(cx<T>);
+    MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    dim3 grid((unsigned)((nrows + RB - 1) / RB));
+    ProfScope prof(ctx, std::string(name) + "_n" + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   (double)nrows * (NI + NO) * kc * sizeof(cx<T>));
+    hipLaunchKernelGGL((k_rows_fused<T, M, PR>), grid, dim3(RB * NI * SL), lds, ctx->stream,
+                       static_cast<const cx<T>*>(in), stride, static_cast<cx<T>*>(out), stride, kc, nrows, (T)scale,
+                       tw);
+    MFIC_LAUNCH_CHECK();
+}
+
+#define MFIC_FUSED_SIZES(X) X(12) X(18) X(24) X(36) X(48) X(72) X(96) X(144) X(192) X(288) X(384) X(576) X(768)
+
+template <typename T, class PR>
+static void fused_t(mfic_ctx ctx, size_t n, const void* in, void* out, const char* name) {
+    const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
+    const long stride = (long)np * np * kc;
+    const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
+    switch (np) {
+#define X(S)                                                                              \
+    case S:                                                                               \
+        launch_fused<T, S / 2, PR>(ctx, in, out, stride, kc, (long)np * np, norm, name); \
+        break;
+        MFIC_FUSED_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
+    }
+}
+
+void fused_source_rows(mfic_ctx ctx, int prec, FusedKind kind, size_t n, const void* in, void* out) {
+    const bool f64 = prec == MFIC_F64;
+    switch (kind) {
+    case FusedKind::Src2:
+        f64 ? fused_t<double, ProdSrc2>(ctx, n, in, out, "fused_src2") : fused_t<float, ProdSrc2>(ctx, n, in, out, "fused_src2");
+        break;
+    case FusedKind::Src2Q:
+        f64 ? fused_t<double, ProdSrc2Q>(ctx, n, in, out, "fused_src2q") : fused_t<float, ProdSrc2Q>(ctx, n, in, out, "fused_src2q");
+        break;
+    case FusedKind::Src3:
+        f64 ? fused_t<double, ProdSrc3>(ctx, n, in, out, "fused_src3") : fused_t<float, ProdSrc3>(ctx, n, in, out, "fused_src3");
+        break;
+    }
+}
+
+int fused_inputs(FusedKind kind) { return kind == FusedKind::Src3 ? ProdSrc3::NI : ProdSrc2::NI; }
+int fused_outputs(FusedKind kind) {
+    return kind == FusedKind::Src3 ? ProdSrc3::NO : kind == FusedKind::Src2Q ? ProdSrc2Q::NO : ProdSrc2::NO;
+}
+
+}  // namespace mfic

@@ -6,12 +6,14 @@
 //    operands, two c2r, product, r2c, unpad + output operator), kept for the
 //    drop-in OrszagConvolver interface;
 //  * lpt_sources: the reorganised schedule used by the pipeline.  Each
-//    Hessian of phi1 and phi2 is transformed to padded real space once, all
-//    products sharing one truncation are summed (in FP64) before a single
-//    r2c, and the cubic 3a terms keep their two-stage truncation
-//    trunc(trunc(a b) c) (convolution.hh:525-532) with the inner products
-//    grouped by their outer factor.  Equal to the reference schedule in exact
-//    arithmetic (every step is linear in the grouped quantity).
+//    Hessian of phi1 and phi2 goes through the axis-0 / axis-1 padded c2r
+//    passes once; one fused row pass per LPT order brings all of them to
+//    real space in LDS, sums every product sharing one truncation (in FP64)
+//    and returns the r2c'd rows, so padded real fields never reach HBM.  The
+//    cubic 3a terms keep their two-stage truncation trunc(trunc(a b) c)
+//    (convolution.hh:525-532) with the inner products grouped by their outer
+//    factor.  Equal to the reference schedule in exact arithmetic (every step
+//    is linear in the grouped quantity).
 #include <algorithm>
 #include <cmath>
 
@@ -19,6 +21,7 @@
 #include "kspace.hpp"
 #include "mfic_internal.hpp"
 #include "orszag.hpp"
+#include "fused.hpp"
 #include "pruned.hpp"
 
 namespace mfic {
@@ -262,72 +265,40 @@ void lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_g
     const size_t n = phi1->n[0];
     const double L = phi1->L[0];
     const int prec = phi1->prec;
-    const size_t rb = C.real_reals() * C.esize();
-    void* kbuf = ctx_scratch(ctx, "kbuf", C.kbuf_reals() * C.esize());
+    // compact padded half-spectrum buffers, consecutive: inputs of the fused
+    // row pass (phi1 Hessians, phi2 Hessians, Q) and its outputs
+    const size_t kb = C.kbuf_reals() * C.esize();
+    const FusedKind k1 = order >= 3 ? FusedKind::Src2Q : FusedKind::Src2;
+    char* kin = static_cast<char*>(ctx_scratch(ctx, "fused_in", (size_t)(order >= 3 ? 16 : 6) * kb));
+    char* kout = static_cast<char*>(ctx_scratch(ctx, "fused_out", (size_t)fused_outputs(k1) * kb));
     // xx yy zz xy xz yz
     static const int HD[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
-    auto hidx = [](int a, int b) {
-        if (a > b) std::swap(a, b);
-        if (a == b) return a;
-        return a == 0 ? (b == 1 ? 3 : 4) : 5;
-    };
-    void* H1[6];
-    for (int h = 0; h < 6; ++h) {
-        H1[h] = ctx_scratch(ctx, "H1_" + std::to_string(h), rb);
-        padded_c2r_from_operand(ctx, prec, OperandDesc{phi1->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
-                                kbuf, H1[h]);
-    }
-    // phi2 source (ic_generator.cc:536-541), then -1/k^2 (ic_generator.cc:554)
-    padded_r2c_products(ctx, prec,
-                        {{H1[0], H1[1], 1.0}, {H1[0], H1[2], 1.0}, {H1[1], H1[2], 1.0},
-                         {H1[3], H1[3], -1.0}, {H1[4], H1[4], -1.0}, {H1[5], H1[5], -1.0}},
-                        n, L, kbuf, phi2->d, MFIC_OP_ASSIGN, 0.0, true);
+    for (int h = 0; h < 6; ++h)
+        padded_c2r_to_kbuf(ctx, prec, OperandDesc{phi1->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
+                           kin + h * kb);
+    // phi2 source (ic_generator.cc:536-541) and, for 3LPT, the inner 3a
+    // products Qzz Qyz Qxx Qyy (ic_generator.cc:581-585) from one row pass
+    fused_source_rows(ctx, prec, k1, n, kin, kout);
+    padded_r2c_from_kbuf(ctx, prec, n, L, kout, phi2->d, MFIC_OP_ASSIGN, 0.0, true);  // -1/k^2: ic_generator.cc:554
     phi2->space = MFIC_KSPACE;
     if (order < 3) return;
-    void* H2[6];
-    for (int h = 0; h < 6; ++h) {
-        H2[h] = ctx_scratch(ctx, "H2_" + std::to_string(h), rb);
-        padded_c2r_from_operand(ctx, prec, OperandDesc{phi2->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
-                                kbuf, H2[h]);
-    }
-    // 3a inner products, truncated once and brought back to padded real space:
-    // Qzz = xx yy - xy^2 (outer zz), Qyz = xy xz (outer yz, x2), Qxx = yz^2
-    // (outer xx, -1), Qyy = xz^2 (outer yy, -1)   (ic_generator.cc:581-585)
-    const std::vector<ProdTerm> inner[4] = {{{H1[0], H1[1], 1.0}, {H1[3], H1[3], -1.0}},
-                                            {{H1[3], H1[4], 1.0}},
-                                            {{H1[5], H1[5], 1.0}},
-                                            {{H1[4], H1[4], 1.0}}};
-    void* Q[4];
+    // Q: truncated once (first stage of trunc(trunc(a b) c), convolution.hh:
+    // 525-532) and padded again for the outer products
     void* qk = ctx_scratch(ctx, "qk", C.grid_reals() * C.esize());
     for (int q = 0; q < 4; ++q) {
-        Q[q] = ctx_scratch(ctx, "Q_" + std::to_string(q), rb);
-        padded_r2c_products(ctx, prec, inner[q], n, L, kbuf, qk, MFIC_OP_ASSIGN, 0.0, false);
-        padded_c2r_from_operand(ctx, prec, OperandDesc{qk, MFIC_OPND_IDENT, {0, 0, 0, 0}}, n, L, kbuf, Q[q]);
+        padded_r2c_from_kbuf(ctx, prec, n, L, kout + (1 + q) * kb, qk, MFIC_OP_ASSIGN, 0.0, false);
+        padded_c2r_to_kbuf(ctx, prec, OperandDesc{qk, MFIC_OPND_IDENT, {0, 0, 0, 0}}, n, L, kin + (12 + q) * kb);
     }
-    // phi3 source: 3a outer products + 3b (ic_generator.cc:581-609)
-    {
-        const double a = -5.0 / 7.0, b = 10.0 / 7.0;
-        padded_r2c_products(ctx, prec,
-                            {{Q[0], H1[2], 1.0}, {Q[1], H1[5], 2.0}, {Q[2], H1[0], -1.0}, {Q[3], H1[1], -1.0},
-                             {H1[0], H2[1], a}, {H1[0], H2[2], a}, {H1[1], H2[2], a}, {H1[1], H2[0], a},
-                             {H1[2], H2[0], a}, {H1[2], H2[1], a},
-                             {H1[3], H2[3], b}, {H1[4], H2[4], b}, {H1[5], H2[5], b}},
-                            n, L, kbuf, phi3->d, MFIC_OP_ASSIGN, 0.0, true);
-        phi3->space = MFIC_KSPACE;
-    }
-    // A3 (ic_generator.cc:614-627)
-    mfic_grid A3[3] = {A3x, A3y, A3z};
-    for (int d = 0; d < 3; ++d) {
-        const int dp = (d + 1) % 3, dpp = (d + 2) % 3;
-        padded_r2c_products(ctx, prec,
-                            {{H2[hidx(d, dp)], H1[hidx(d, dpp)], 1.0},
-                             {H2[hidx(d, dpp)], H1[hidx(d, dp)], -1.0},
-                             {H1[hidx(dp, dpp)], H2[hidx(dp, dp)], 1.0},
-                             {H1[hidx(dp, dpp)], H2[hidx(dpp, dpp)], -1.0},
-                             {H2[hidx(dp, dpp)], H1[hidx(dp, dp)], -1.0},
-                             {H2[hidx(dp, dpp)], H1[hidx(dpp, dpp)], 1.0}},
-                            n, L, kbuf, A3[d]->d, MFIC_OP_ASSIGN, 0.0, true);
-        A3[d]->space = MFIC_KSPACE;
+    for (int h = 0; h < 6; ++h)
+        padded_c2r_to_kbuf(ctx, prec, OperandDesc{phi2->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
+                           kin + (6 + h) * kb);
+    // phi3 source (3a outer + 3b, ic_generator.cc:581-609) and A3
+    // (ic_generator.cc:614-627), then -1/k^2 each
+    fused_source_rows(ctx, prec, FusedKind::Src3, n, kin, kout);
+    mfic_grid outs[4] = {phi3, A3x, A3y, A3z};
+    for (int o = 0; o < 4; ++o) {
+        padded_r2c_from_kbuf(ctx, prec, n, L, kout + o * kb, outs[o]->d, MFIC_OP_ASSIGN, 0.0, true);
+        outs[o]->space = MFIC_KSPACE;
     }
 }
 

@@ -209,36 +209,36 @@ template <typename T, int M, int RW>
 __global__ void __launch_bounds__(Plan<M>::P * RW)
 k_rows_c2r_pruned(const cx<T>* __restrict__ kin, int kc, T* __restrict__ rout, long rpitch, long nrows, T scale,
                   const cx<T>* __restrict__ tw) {
-    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = row_lds_pitch<M>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int q = threadIdx.x % P, rw = threadIdx.x / P;
     const long row = (long)blockIdx.x * RW + rw;
     const bool active = row < nrows;
     const cx<T>* src = kin + (active ? row : 0) * kc;
-    cx<T>* l = lds + rw * RP;
+    const LdsRowMap lm{rw * RP};
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int k = q + P * e;
         cx<T> x = (active && k < kc) ? src[k] : cx<T>{0, 0};
         if (k == 0) x.y = 0;
-        l[k] = x;
+        lds[lm.at(k)] = x;
     }
-    if (q == 0) l[M] = (active && M < kc) ? cx<T>{src[M].x, (T)0} : cx<T>{0, 0};
+    if (q == 0) lds[lm.at(M)] = (active && M < kc) ? cx<T>{src[M].x, (T)0} : cx<T>{0, 0};
     __syncthreads();
     const cx<T>* uw = tw + plan_tw_size<M>();
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int k = q + P * e;
-        const cx<T> xk = l[k], xm = l[M - k];
+        const cx<T> xk = lds[lm.at(k)], xm = lds[lm.at(M - k)];
         const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
         const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
         const cx<T> B = cmulc(D, uw[k]);
         v[e] = cx<T>{A.x - B.y, A.y + B.x};
     }
     __syncthreads();
-    fft_regs<T, M, false>(v, q, lds, LdsMap{rw * RP, 1}, tw);
+    fft_regs<T, M, false>(v, q, lds, LdsRowMap{rw * RP}, tw);
     if (active) {
         cx<T>* dst = reinterpret_cast<cx<T>*>(rout + row * rpitch);
 #pragma unroll
@@ -259,7 +259,7 @@ template <typename T, int M, int RW>
 __global__ void __launch_bounds__(Plan<M>::P * RW)
 k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long nrows, T scale,
                 const cx<T>* __restrict__ tw) {
-    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = M + 1;
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, RP = row_lds_pitch<M>();
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int q = threadIdx.x % P, rw = threadIdx.x / P;
@@ -292,10 +292,10 @@ k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long 
 #pragma unroll
         for (int u = 0; u < CH; ++u) v[e0 + u] = cx<T>{(T)s0[u], (T)s1[u]};
     }
-    fft_regs<T, M, true>(v, q, lds, LdsMap{rw * RP, 1}, tw);
-    cx<T>* l = lds + rw * RP;
+    fft_regs<T, M, true>(v, q, lds, LdsRowMap{rw * RP}, tw);
+    const LdsRowMap lm{rw * RP};
 #pragma unroll
-    for (int e = 0; e < E; ++e) l[q + P * e] = v[e];
+    for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
     __syncthreads();
     const cx<T>* uw = tw + plan_tw_size<M>();
     if (active) {
@@ -304,13 +304,13 @@ k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long 
         for (int e = 0; e < E; ++e) {
             const int k = q + P * e;
             if (k >= kc) continue;
-            const cx<T> zk = v[e], zm = l[k == 0 ? 0 : M - k];
+            const cx<T> zk = v[e], zm = lds[lm.at(k == 0 ? 0 : M - k)];
             const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
             const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
             dst[k] = cscale(cadd(Ek, cmul(uw[k], Ok)), scale);
         }
         if (q == 0 && M < kc) {
-            const cx<T> z0 = l[0];
+            const cx<T> z0 = lds[lm.at(0)];
             dst[M] = cx<T>{(z0.x - z0.y) * scale, (T)0};
         }
     }
@@ -360,7 +360,7 @@ static int pick_rows(mfic_ctx ctx, bool r2c) {
     rw = p2;
     const auto ov = ctx->tiles.find(std::make_tuple(M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) rw = ov->second.first;
-    while (rw > 1 && ((size_t)(M + 1) * rw * sizeof(cx<T>) > 139264 || P * rw > 1024)) rw /= 2;
+    while (rw > 1 && ((size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>) > 139264 || P * rw > 1024)) rw /= 2;
     return rw;
 }
 
@@ -369,7 +369,7 @@ static void rows_c2r_M(mfic_ctx ctx, const cx<T>* kin, int kc, T* rout, long rpi
     constexpr int P = Plan<M>::P;
     const int rw = pick_rows<T, M>(ctx, false);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
-    const size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
+    const size_t lds = (size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>);
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
     ProfScope prof(ctx, std::string("fft_c2rC_n") + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    (double)nrows * ((double)kc * sizeof(cx<T>) + 2.0 * M * sizeof(T)));
@@ -393,7 +393,7 @@ static void rows_r2c_M(mfic_ctx ctx, const RowProd& pr, long rpitch, cx<T>* kout
     constexpr int P = Plan<M>::P;
     const int rw = pick_rows<T, M>(ctx, true);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
-    const size_t lds = (size_t)(M + 1) * rw * sizeof(cx<T>);
+    const size_t lds = (size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>);
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
     ProfScope prof(ctx, std::string("fft_r2cC_n") + std::to_string(2 * M) + "_a" + std::to_string(pr.narr) +
                             (sizeof(T) == 8 ? "_f64" : "_f32"),
@@ -485,6 +485,66 @@ static void r2c_products_t(mfic_ctx ctx, const std::vector<ProdTerm>& terms, siz
     default:
         throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
     }
+}
+
+template <typename T>
+static void c2r_to_kbuf_t(mfic_ctx ctx, const OperandDesc& o, size_t n, double L, void* kbuf) {
+    const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
+    const double rfac = std::pow(1.5, 1.5);
+    const double kf = 2.0 * 3.14159265358979323846 / L;
+    LdOperand<T> la{static_cast<const cx<T>*>(o.f), N, kc, o.kind, o.d[0], o.d[1], o.d[2], o.d[3], kf, rfac};
+    StPadCol<T> sa{static_cast<cx<T>*>(kbuf), N, np, kc};
+    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 1};
+    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 0};
+    const double cs = sizeof(cx<T>);
+    const long colsA = (long)N * kc, colsB = (long)np * kc;
+    switch (np) {
+#define X(S)                                                                                   \
+    case S:                                                                                    \
+        launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs); \
+        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs); \
+        break;
+        MFIC_PADDED_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
+    }
+}
+
+template <typename T>
+static void r2c_from_kbuf_t(mfic_ctx ctx, size_t n, double L, void* kbuf, void* out, int op, double coef, bool invlap) {
+    const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
+    const double rfac = std::sqrt((double)np * np * np) / std::sqrt((double)N * N * N);
+    const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
+    const double kf = 2.0 * 3.14159265358979323846 / L;
+    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 0};
+    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 1};
+    LdPadCol<T> la{static_cast<const cx<T>*>(kbuf), N, np, kc};
+    StUnpad<T> sa{static_cast<cx<T>*>(out), N, kc, op, invlap ? 1 : 0, norm / rfac, coef, kf};
+    const double cs = sizeof(cx<T>);
+    const long colsA = (long)N * kc, colsB = (long)np * kc;
+    switch (np) {
+#define X(S)                                                                                                        \
+    case S:                                                                                                         \
+        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs);                            \
+        launch_f<T, S, true>(ctx, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
+        break;
+        MFIC_PADDED_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
+    }
+}
+
+void padded_c2r_to_kbuf(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf) {
+    if (prec == MFIC_F64) c2r_to_kbuf_t<double>(ctx, o, n, L, kbuf);
+    else c2r_to_kbuf_t<float>(ctx, o, n, L, kbuf);
+}
+
+void padded_r2c_from_kbuf(mfic_ctx ctx, int prec, size_t n, double L, void* kbuf, void* out, int op, double coef,
+                          bool invlap) {
+    if (prec == MFIC_F64) r2c_from_kbuf_t<double>(ctx, n, L, kbuf, out, op, coef, invlap);
+    else r2c_from_kbuf_t<float>(ctx, n, L, kbuf, out, op, coef, invlap);
 }
 
 void padded_c2r_from_operand(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf, void* rout) {

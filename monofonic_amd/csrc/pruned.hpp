@@ -21,4 +21,13 @@ void padded_c2r_from_operand(mfic_ctx ctx, int prec, const OperandDesc& o, size_
 void padded_r2c_products(mfic_ctx ctx, int prec, const std::vector<ProdTerm>& terms, size_t n, double L, void* kbuf,
                          void* out, int op, double coef, bool invlap);
 
+// operand -> compact padded buffer after the axis-0 and axis-1 c2r passes
+// (half-spectrum rows ready for a row pass)
+void padded_c2r_to_kbuf(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf);
+// compact padded buffer of r2c'd rows -> axis-1 and axis-0 r2c passes with the
+// unpad rules, output operator and optional -1/k^2 into the N-grid; kbuf is
+// overwritten
+void padded_r2c_from_kbuf(mfic_ctx ctx, int prec, size_t n, double L, void* kbuf, void* out, int op, double coef,
+                          bool invlap);
+
 }  // namespace mfic

@@ -113,6 +113,28 @@ def test_potentials_match_oracle(ctx, order, prec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [24, 64, 96, 192, 256])
+def test_potentials_3lpt_sizes(ctx, n):
+    """3LPT potentials across padded row lengths: one / several transforms
+    per wavefront, lane slots wider than a transform (96^3, 192^3) and the
+    16-lane transforms of 256^3"""
+    box, z, seed = 2.0 * n, 24.0, 99
+    cfg = O.ic_cfg(n, box, 3, seed)
+    oc = O.baseline_cosmology(z)
+    pots = O.potentials(cfg, oc, O.ngenic_fill(seed, n))
+    coef = O.lpt_coefs(cfg, oc)
+    run = lpt.ICRun(ctx, lpt.ICConfig(n=n, box=box, zstart=z, lpt_order=3, prec=F64, seed=seed))
+    run.execute()
+    scale = [coef.g1, coef.g2, coef.g3, coef.g3c, coef.g3c, coef.g3c]
+    refs = [pots[0], pots[1], pots[2]] + list(pots[3])
+    for w in range(6):
+        err = rel_l2(run.potential(w).download_k() * scale[w], O.kview(refs[w]))
+        print(f"n {n} potential {w}: rel-L2 {err:.3e}")
+        assert err < 50 * TOL[F64], (w, err)
+    run.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("prec", [F64, F32])
 def test_fields_match_oracle(ctx, order, prec):
