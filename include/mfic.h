@@ -184,11 +184,12 @@ int mfic_conv_convolve3(mfic_conv c, const mfic_operand *a, const mfic_operand *
 /* ------------------------------------------------------- LPT potentials --- */
 /* The LPT source block of ic_generator::run (ic_generator.cc:523-629):
  * phi2 (2LPT), phi3 and A3x/y/z (3LPT) from phi1, each followed by the
- * inverse Laplacian.  Computed with the reorganised schedule: each Hessian is
- * transformed to padded real space once, products sharing a truncation are
- * summed before one r2c, the cubic terms keep their two-stage truncation.
- * Growth factors are NOT applied here (see mfic_lpt_scale).  Grids not needed
- * for the order may be NULL. */
+ * inverse Laplacian.  Computed with the reorganised schedule: each Hessian
+ * goes through the padded axis-0/axis-1 c2r passes once, one fused row pass
+ * per order forms every product sharing a truncation (FP64 sums) in LDS and
+ * returns r2c'd rows, the cubic terms keep their two-stage truncation.
+ * Growth factors are NOT applied here (mfic_lpt_component applies them).
+ * Grids not needed for the order may be NULL. */
 int mfic_lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_grid phi3,
                      mfic_grid A3x, mfic_grid A3y, mfic_grid A3z);
 
