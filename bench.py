@@ -8,9 +8,10 @@ work (seed table, PLT tables, cosmology scalars) runs once before timing.
 
 Prints one JSON line.  Extra objects: "roofline" (dominant kernel, HIP-event
 timing on the library stream during the timed steps), "cpu_baseline" (the
-CPU FP64 oracle on a bounded 128^3 sample of the same configuration),
-"fp64" (same workload with FP64 bulk) and "accuracy" (FP32 vs FP64 device
-fields, rel-L2 / max error per component).
+CPU FP64 oracle on a bounded 256^3 sample of the same configuration),
+"fp64" (same workload with FP64 bulk), "accuracy" (FP32 vs FP64 device
+fields, rel-L2 / max error per component) and "batch" (48 independent 2LPT
+boxes of 128^3-256^3 replayed as one HIP graph with autotuned tiles).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
        python bench.py --calibrate-copy          (copy kernel for counter scaling)
@@ -156,6 +157,48 @@ def cpu_baseline(n_sample=256):
             "seconds": round(t, 3)}
 
 
+def batch_bench(reps=3):
+    """48 independent 2LPT FP32 boxes (16 x 128^3, 16 x 192^3, 16 x 256^3,
+    seeds 12345+i, box sizes from mt19937_64(20261202)), tiles autotuned per
+    size, replayed as one captured HIP graph; compared with the same boxes
+    enqueued one after another and with one host-synchronised run per box"""
+    from monofonic_amd import F32, Context
+    from monofonic_amd.batch import Batch, batch_boxes
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    b = Batch(ctx, batch_boxes(), lpt_order=2, prec=F32, zstart=24.0, tune=True)
+    b.execute()
+    plan_s = time.perf_counter() - t0
+
+    def timed(fn):
+        device_sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        device_sync()
+        return (time.perf_counter() - t) / reps
+
+    seq = timed(lambda: b.execute(blocking=False))
+    per_box = timed(lambda: [r.execute(blocking=True) for r in b.runs])
+    b.capture()
+    b.launch()
+    graph = timed(lambda: b.launch(blocking=False))
+    cells = b.cells
+    out = {"workload": "48 boxes 2LPT FP32 fields (16x128^3, 16x192^3, 16x256^3), seeds 12345+i, "
+                       "L_i = 100 + 700 U_i (mt19937_64(20261202)), z_start 24",
+           "cells": cells, "value": round(cells / graph, 1), "unit": "cells/s",
+           "graph_ms": round(graph * 1e3, 3), "sequential_ms": round(seq * 1e3, 3),
+           "sequential_value": round(cells / seq, 1), "synchronised_per_box_ms": round(per_box * 1e3, 3),
+           "synchronised_per_box_value": round(cells / per_box, 1),
+           "graph_speedup_vs_synchronised": round(per_box / graph, 3),
+           "autotuned_ms_per_box": {str(k): round(v, 3) for k, v in b.tuned.items()},
+           "plan_and_tune_s": round(plan_s, 2)}
+    b.close()
+    ctx.close()
+    return out
+
+
 def accuracy(ctx, n, box):
     """FP32 vs FP64 device component fields (same seed): rel-L2 and max error"""
     import numpy as np
@@ -250,6 +293,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box of this workload fits one "
@@ -294,6 +338,9 @@ def main():
         log("accuracy: FP32 vs FP64 device fields")
         line["accuracy"] = {"reference": "FP64 device run (same seed, same schedule)", **accuracy(ctx, n, box)}
     ctx.close()
+    if not args.no_batch:
+        log("batched boxes: 48 x 2LPT FP32 as one HIP graph")
+        line["batch"] = batch_bench()
     if not args.no_cpu:
         log("CPU baseline (oracle) on a 256^3 sample")
         line["cpu_baseline"] = cpu_baseline()

@@ -263,6 +263,31 @@ int mfic_run_particles(mfic_run r, double **pos, double **vel, size_t *npart);
  * growth scaling: the assembly applies g1..g3c (ic_generator.cc:644-658) */
 int mfic_run_potential(mfic_run r, int which, mfic_grid *out);
 
+/* -------------------------------------------------------------- batching --- */
+/* No reference counterpart: monofonIC runs one box per process, and FFTW
+ * planning (grid_fft.cc:53-54) plays the autotuner's role.
+ *
+ * Graph capture: every call enqueued on the context's stream between
+ * mfic_graph_begin and mfic_graph_end (for example mfic_run_execute of many
+ * runs of different sizes and seeds) is recorded into one HIP graph, replayed
+ * by mfic_graph_launch.  Whatever the captured calls allocate lazily (scratch
+ * buffers, twiddle tables) must already exist: execute the same sequence once
+ * before capturing.  A call that would allocate fails with MFIC_ERR_STATE and
+ * mfic_graph_end then reports the broken capture.  Per-kernel timing is
+ * suspended while capturing. */
+typedef struct mfic_graph_s *mfic_graph;
+int mfic_graph_begin(mfic_ctx ctx);
+int mfic_graph_end(mfic_ctx ctx, mfic_graph *out);
+int mfic_graph_launch(mfic_graph g, int blocking);
+int mfic_graph_destroy(mfic_graph g);
+/* Tile autotuner: for a box of size n, precision prec and LPT order, times
+ * whole pipeline executions over the pencils-per-block choices of the N-grid
+ * and padded strided passes and the rows-per-block choices of the N-grid
+ * contiguous pass, keeping the fastest per transform length in the context
+ * (mfic_fft_set_tile overrides).  best_ms (may be NULL) receives the tuned
+ * time of one execution. */
+int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

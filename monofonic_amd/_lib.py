@@ -118,6 +118,11 @@ SIGNATURES = {
     "mfic_run_field": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "mfic_run_particles": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_size_t)]),
     "mfic_run_potential": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "mfic_graph_begin": (C.c_int, [_P]),
+    "mfic_graph_end": (C.c_int, [_P, C.POINTER(_P)]),
+    "mfic_graph_launch": (C.c_int, [_P, C.c_int]),
+    "mfic_graph_destroy": (C.c_int, [_P]),
+    "mfic_autotune": (C.c_int, [_P, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -35,6 +35,8 @@ template <typename F> static int guard(F&& f) {
 void* ctx_scratch(mfic_ctx ctx, const std::string& key, size_t bytes) {
     auto& s = ctx->scratch[key];
     if (s.second < bytes) {
+        MFIC_CHECK(!ctx->capturing, MFIC_ERR_STATE,
+                   "scratch '" + key + "' would grow during graph capture: run the sequence once before capturing");
         if (s.first) MFIC_HIP(hipFree(s.first));
         s.first = nullptr;
         s.second = 0;
@@ -57,7 +59,7 @@ static hipEvent_t pool_event(mfic_ctx ctx) {
 }
 
 ProfScope::ProfScope(mfic_ctx ctx, const std::string& name, double bytes) : ctx_(ctx) {
-    if (!ctx_ || !ctx_->prof) return;
+    if (!ctx_ || !ctx_->prof || ctx_->capturing) return;
     mfic_prof_rec r{name, bytes, pool_event(ctx_), pool_event(ctx_)};
     MFIC_HIP(hipEventRecord(r.a, ctx_->stream));
     idx_ = (long)ctx_->prof_recs.size();
@@ -90,6 +92,10 @@ int mfic_ctx_create(int device, mfic_ctx* out) {
         MFIC_HIP(hipSetDevice(device));
         auto* c = new mfic_ctx_s();
         c->device = device;
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lds <= 0)
+            lds = 65536;
+        c->max_lds = (size_t)lds;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete c;

@@ -13,9 +13,6 @@
 #include "ngenic.hpp"
 #include "orszag.hpp"
 
-namespace mfic {
-template <typename F> int guard_call(F&& f);
-}
 
 struct mfic_conv_s {
     std::unique_ptr<mfic::ConvCore> core;
@@ -50,25 +47,6 @@ struct mfic_run_s {
 
 namespace mfic {
 
-
-template <typename F> int guard_call(F&& f) {
-    try {
-        f();
-        return MFIC_OK;
-    } catch (const Error& e) {
-        set_last_error(e.what());
-        return e.code;
-    } catch (const std::invalid_argument& e) {
-        set_last_error(e.what());
-        return MFIC_ERR_CONFIG;
-    } catch (const std::bad_alloc&) {
-        set_last_error("host allocation failed");
-        return MFIC_ERR_ALLOC;
-    } catch (const std::exception& e) {
-        set_last_error(e.what());
-        return MFIC_ERR_ARG;
-    }
-}
 
 static void need(mfic_grid g, const char* what) {
     MFIC_CHECK(g && g->d, MFIC_ERR_ARG, std::string("null or unallocated grid: ") + what);

@@ -175,6 +175,8 @@ const void* twiddles(mfic_ctx ctx, int L, int kind, int prec) {
     auto key = std::make_tuple(L, kind, prec);
     auto it = ctx->tables.find(key);
     if (it != ctx->tables.end()) return it->second.dev;
+    MFIC_CHECK(!ctx->capturing, MFIC_ERR_STATE,
+               "twiddle table for length " + std::to_string(L) + " created during graph capture: run the sequence once before capturing");
     auto tw = make_table(L, kind);
     TwTable t;
     t.count = tw.size();
@@ -269,7 +271,7 @@ static void launch_rows_M(mfic_ctx ctx, T* data, size_t nrows, long pitch, T sca
     int rw = std::max(1, std::min(64, pow2_floor((R2C ? 512 : 256) / P / (M >= 256 ? 4 : 1))));
     const auto ov = ctx->tiles.find(std::make_tuple(M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) rw = ov->second.first;
-    while (rw > 1 && (size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>) > max_lds(ctx)) rw /= 2;
+    while (rw > 1 && ((size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>) > max_lds(ctx) || P * rw > 1024)) rw /= 2;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     dim3 grid((unsigned)((nrows + rw - 1) / rw));
     size_t lds = (size_t)row_lds_pitch<M>() * rw * sizeof(cx<T>);

@@ -70,8 +70,11 @@ template <int M, int NI> constexpr int fused_rows_per_block() {
     return tpr >= 256 ? 1 : (256 / tpr > 8 ? 8 : 256 / tpr);
 }
 
+// FP32: ask for three waves per SIMD (at most 168 VGPRs), which the LDS of
+// a 16-input row (52 KB at 768) also allows
 template <typename T, int M, class PR>
 __global__ void __launch_bounds__((fused_rows_per_block<M, PR::NI>() * PR::NI * lane_slot<Plan<M>::P>()))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8)))
 k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out, long ostride, int kc, long nrows,
              T scale, const cx<T>* __restrict__ tw) {
     constexpr int NI = PR::NI, NO = PR::NO, E = Plan<M>::E, P = Plan<M>::P, SL = lane_slot<P>();

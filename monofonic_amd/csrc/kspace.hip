@@ -98,6 +98,36 @@ __global__ void k_phi1(cx<T>* __restrict__ phi, const cx<T>* __restrict__ wn, KG
     phi[idx] = cx<T>{(T)pr, (T)pi};
 }
 
+// For a cubic grid with equal box sides |k|^2 = n2 kf^2 with the integer
+// n2 = i'^2 + j'^2 + k'^2, so the FP64 transfer-function factor -A(k)/k^2 is
+// tabulated once per distinct n2 (3 (N/2)^2 + 1 entries) instead of per mode.
+__global__ void k_phi1_table(double* __restrict__ tab, int nmax, double kf, EhParams eh) {
+    const int n2 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n2 > nmax) return;
+    if (n2 == 0) { tab[0] = 0.0; return; }
+    const double k2 = (double)n2 * kf * kf;
+    tab[n2] = -eh_amplitude(eh, sqrt(k2)) / k2;
+}
+
+template <typename T>
+__global__ void k_phi1_tab(cx<T>* __restrict__ phi, const cx<T>* __restrict__ wn, const double* __restrict__ tab,
+                           KGeom G, size_t ncplx, double volfac, int fixing, int inversion) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ncplx) return;
+    int i, j, k;
+    G.ijk(idx, i, j, k);
+    const cx<T> w = wn[idx];
+    double wr = w.x, wi = w.y;
+    if (fixing) {
+        const double a = hypot(wr, wi);
+        if (a != 0.0) { wr /= a; wi /= a; }
+    }
+    const double sg = inversion ? -1.0 : 1.0;
+    const int si = i - (i > G.n0 / 2) * G.n0, sj = j - (j > G.n1 / 2) * G.n1, sk = k - (k > G.n2 / 2) * G.n2;
+    const double F = tab[si * si + sj * sj + sk * sk];
+    phi[idx] = cx<T>{(T)(sg * wr / volfac * F), (T)(sg * wi / volfac * F)};
+}
+
 // ----------------------------------------------------------- launchers ---
 
 template <typename T> static T* dptr(mfic_grid g) { return static_cast<T*>(g->d); }
@@ -170,6 +200,21 @@ void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double vol
     const size_t nc = (size_t)G.n0 * G.n1 * G.nc;
     const int bs = 256;
     ProfScope prof(phi->ctx, "phi1", 2.0 * phi->bytes());
+    if (G.n0 == G.n1 && G.n1 == G.n2 && G.kf0 == G.kf1 && G.kf1 == G.kf2) {
+        const int nmax = 3 * (G.n0 / 2) * (G.n0 / 2);
+        double* tab = static_cast<double*>(ctx_scratch(phi->ctx, "phi1_tab", (size_t)(nmax + 1) * sizeof(double)));
+        hipLaunchKernelGGL(k_phi1_table, dim3(nblocks((size_t)nmax + 1, bs)), dim3(bs), 0, phi->ctx->stream, tab, nmax,
+                           G.kf0, eh);
+        MFIC_LAUNCH_CHECK();
+        if (phi->prec == MFIC_F64)
+            hipLaunchKernelGGL(k_phi1_tab<double>, dim3(nblocks(nc, bs)), dim3(bs), 0, phi->ctx->stream,
+                               (cx<double>*)phi->d, (const cx<double>*)wn->d, tab, G, nc, volfac, fixing, inversion);
+        else
+            hipLaunchKernelGGL(k_phi1_tab<float>, dim3(nblocks(nc, bs)), dim3(bs), 0, phi->ctx->stream,
+                               (cx<float>*)phi->d, (const cx<float>*)wn->d, tab, G, nc, volfac, fixing, inversion);
+        MFIC_LAUNCH_CHECK();
+        return;
+    }
     if (phi->prec == MFIC_F64)
         hipLaunchKernelGGL(k_phi1<double>, dim3(nblocks(nc, bs)), dim3(bs), 0, phi->ctx->stream, (cx<double>*)phi->d,
                            (const cx<double>*)wn->d, G, nc, eh, volfac, fixing, inversion);

@@ -62,6 +62,9 @@ struct mfic_ctx_s {
     size_t max_lds = 0;  // LDS bytes per workgroup (device attribute)
     // tile overrides from the autotuner: (length, kind, prec) -> (tile, lds pad)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tiles;
+    // stream capture into a HIP graph in progress (mfic_graph_begin): no
+    // allocation, table creation or event timing may happen meanwhile
+    bool capturing = false;
     // per-launch event timing (mfic_prof_enable)
     bool prof = false;
     std::vector<mfic_prof_rec> prof_recs;
@@ -118,5 +121,26 @@ template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int
 template <typename T>
 void fft_pencils(mfic_ctx ctx, void* data, int L, long ps, long os, int tlen, long nouter, bool fwd,
                  double scale, const char* tag = "");
+
+// run f, translating exceptions into a status code and the thread-local
+// error message (no exception crosses the C ABI)
+template <typename F> int guard_call(F&& f) {
+    try {
+        f();
+        return MFIC_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        set_last_error(e.what());
+        return MFIC_ERR_CONFIG;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return MFIC_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MFIC_ERR_ARG;
+    }
+}
 
 }  // namespace mfic

@@ -271,11 +271,8 @@ void lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_g
     const FusedKind k1 = order >= 3 ? FusedKind::Src2Q : FusedKind::Src2;
     char* kin = static_cast<char*>(ctx_scratch(ctx, "fused_in", (size_t)(order >= 3 ? 16 : 6) * kb));
     char* kout = static_cast<char*>(ctx_scratch(ctx, "fused_out", (size_t)fused_outputs(k1) * kb));
-    // xx yy zz xy xz yz
-    static const int HD[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
-    for (int h = 0; h < 6; ++h)
-        padded_c2r_to_kbuf(ctx, prec, OperandDesc{phi1->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
-                           kin + h * kb);
+    void* xbuf = ctx_scratch(ctx, "hess_x", 3 * kb);
+    padded_hessians_to_kbuf(ctx, prec, phi1->d, n, L, xbuf, kin);
     // phi2 source (ic_generator.cc:536-541) and, for 3LPT, the inner 3a
     // products Qzz Qyz Qxx Qyy (ic_generator.cc:581-585) from one row pass
     fused_source_rows(ctx, prec, k1, n, kin, kout);
@@ -289,9 +286,7 @@ void lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_g
         padded_r2c_from_kbuf(ctx, prec, n, L, kout + (1 + q) * kb, qk, MFIC_OP_ASSIGN, 0.0, false);
         padded_c2r_to_kbuf(ctx, prec, OperandDesc{qk, MFIC_OPND_IDENT, {0, 0, 0, 0}}, n, L, kin + (12 + q) * kb);
     }
-    for (int h = 0; h < 6; ++h)
-        padded_c2r_to_kbuf(ctx, prec, OperandDesc{phi2->d, MFIC_OPND_HESS, {HD[h][0], HD[h][1], 0, 0}}, n, L,
-                           kin + (6 + h) * kb);
+    padded_hessians_to_kbuf(ctx, prec, phi2->d, n, L, xbuf, kin + 6 * kb);
     // phi3 source (3a outer + 3b, ic_generator.cc:581-609) and A3
     // (ic_generator.cc:614-627), then -1/k^2 each
     fused_source_rows(ctx, prec, FusedKind::Src3, n, kin, kout);

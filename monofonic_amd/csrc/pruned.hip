@@ -93,6 +93,49 @@ template <typename T> struct LdOperand {
                       : cx<T>{(T)(s * (double)v.x), (T)(s * (double)v.y)};
     }
 };
+// gradient factor of index idx in the working precision (0 at Nyquist)
+template <typename T> __device__ __forceinline__ T grad_t(int idx, int n, T kf) {
+    return idx == n / 2 ? T(0) : (T)(idx - (idx > n / 2) * n) * kf;
+}
+// pass A of the factorised Hessians: the N-grid field times g_i^p (p = 0..2,
+// the number of x derivatives) and the pad factor
+template <typename T> struct LdPow {
+    const cx<T>* f;
+    int n, kc, p;
+    T kf, scale;
+    struct Col { const cx<T>* p; };
+    __device__ __forceinline__ Col column(long col) const { return Col{f + col}; }
+    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
+        const int i = src_of(pos, n);
+        if (i < 0) return cx<T>{0, 0};
+        const cx<T> v = c.p[(long)i * n * kc];
+        const T g = grad_t<T>(i, n, kf);
+        const T m = p == 0 ? scale : (p == 1 ? scale * g : scale * (g * g));
+        return cx<T>{m * v.x, m * v.y};
+    }
+};
+// pass B of the factorised Hessians: g_j^q along the pencil (zero band
+// skipped) times the per-column factor sign * g_k^r
+template <typename T> struct LdAx1Mul {
+    const cx<T>* kb;
+    int n, np, kc, q, r;
+    T kf, sign;
+    struct Col { const cx<T>* p; T m; };
+    __device__ __forceinline__ Col column(long col) const {
+        const long ip = col / kc;
+        const int k = (int)(col % kc);
+        const T gk = grad_t<T>(k, n, kf);
+        return Col{kb + ip * np * kc + k, r == 0 ? sign : (r == 1 ? sign * gk : sign * (gk * gk))};
+    }
+    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
+        const int j = src_of(pos, n);
+        if (j < 0) return cx<T>{0, 0};
+        const cx<T> v = c.p[(long)pos * kc];
+        const T g = grad_t<T>(j, n, kf);
+        const T m = q == 0 ? c.m : (q == 1 ? c.m * g : c.m * (g * g));
+        return cx<T>{m * v.x, m * v.y};
+    }
+};
 // pass A store: padded column (j' = pad(j)) of the compact k buffer
 template <typename T> struct StPadCol {
     cx<T>* kb;
@@ -534,6 +577,49 @@ static void r2c_from_kbuf_t(mfic_ctx ctx, size_t n, double L, void* kbuf, void* 
     default:
         throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
     }
+}
+
+// The six Hessians (xx yy zz xy xz yz) of an N-grid field, factorised per
+// axis: -g_a g_b = -(g_i^p)(g_j^q)(g_k^r).  Pass A runs once per x power
+// (3 launches into `xbuf`, three consecutive compact buffers), pass B applies
+// the y power along its pencils and the sign and z power per column.
+template <typename T>
+static void hessians_to_kbuf_t(mfic_ctx ctx, const void* f, size_t n, double L, void* xbuf, void* out, long ostride) {
+    const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
+    const double rfac = std::pow(1.5, 1.5);
+    const double kf = 2.0 * 3.14159265358979323846 / L;
+    const long stride = (long)np * np * kc;
+    cx<T>* xb = static_cast<cx<T>*>(xbuf);
+    cx<T>* ob = static_cast<cx<T>*>(out);
+    static const int PQR[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    const double cs = sizeof(cx<T>);
+    const long colsA = (long)N * kc, colsB = (long)np * kc;
+    switch (np) {
+#define X(S)                                                                                                   \
+    case S:                                                                                                    \
+        for (int p = 2; p >= 0; --p) {                                                                         \
+            LdPow<T> la{static_cast<const cx<T>*>(f), N, kc, p, (T)kf, (T)rfac};                               \
+            StPadCol<T> sa{xb + p * stride, N, np, kc};                                                        \
+            launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs);             \
+            for (int h = 0; h < 6; ++h) {                                                                      \
+                if (PQR[h][0] != p) continue;                                                                  \
+                LdAx1Mul<T> lb{xb + p * stride, N, np, kc, PQR[h][1], PQR[h][2], (T)kf, (T)-1};                \
+                StAx1<T> sb{ob + h * ostride, N, np, kc, 0};                                                   \
+                launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs);         \
+            }                                                                                                  \
+        }                                                                                                      \
+        break;
+        MFIC_PADDED_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
+    }
+}
+
+void padded_hessians_to_kbuf(mfic_ctx ctx, int prec, const void* f, size_t n, double L, void* xbuf, void* out) {
+    const long stride = (long)(3 * n / 2) * (3 * n / 2) * (n / 2 + 1);
+    if (prec == MFIC_F64) hessians_to_kbuf_t<double>(ctx, f, n, L, xbuf, out, stride);
+    else hessians_to_kbuf_t<float>(ctx, f, n, L, xbuf, out, stride);
 }
 
 void padded_c2r_to_kbuf(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf) {

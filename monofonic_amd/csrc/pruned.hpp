@@ -24,6 +24,10 @@ void padded_r2c_products(mfic_ctx ctx, int prec, const std::vector<ProdTerm>& te
 // operand -> compact padded buffer after the axis-0 and axis-1 c2r passes
 // (half-spectrum rows ready for a row pass)
 void padded_c2r_to_kbuf(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf);
+// the six Hessians xx yy zz xy xz yz of an N-grid k-space field -> six
+// consecutive compact padded buffers (out) after the axis-0 and axis-1 c2r
+// passes; xbuf is scratch for three compact buffers
+void padded_hessians_to_kbuf(mfic_ctx ctx, int prec, const void* f, size_t n, double L, void* xbuf, void* out);
 // compact padded buffer of r2c'd rows -> axis-1 and axis-0 r2c passes with the
 // unpad rules, output operator and optional -1/k^2 into the N-grid; kbuf is
 // overwritten

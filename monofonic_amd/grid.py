@@ -8,6 +8,7 @@ All work is enqueued on the context stream; ``download`` synchronises.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Sequence
 
 import numpy as np
@@ -22,6 +23,11 @@ class Context:
         self.h = C.c_void_p()
         L.call("mfic_ctx_create", device, C.byref(self.h))
         self.device = device
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj) -> None:
+        """register a handle owner (grid, run, ...) to be closed before the context"""
+        self._children.add(obj)
 
     def sync(self) -> None:
         L.call("mfic_ctx_sync", self.h)
@@ -34,6 +40,8 @@ class Context:
 
     def close(self) -> None:
         if self.h:
+            for child in list(self._children):
+                child.close()
             L.call("mfic_ctx_destroy", self.h)
             self.h = C.c_void_p()
 
@@ -62,6 +70,7 @@ class Grid:
             self.h = C.c_void_p()
             L.call("mfic_grid_create", ctx.h, (C.c_size_t * 3)(*self.n), (C.c_double * 3)(*self.length),
                    prec, space, C.byref(self.h))
+            ctx.adopt(self)
         else:
             self.h = handle
 

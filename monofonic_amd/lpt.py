@@ -91,6 +91,7 @@ class OrszagConvolver:
     def __init__(self, ctx: Context, n: Sequence[int], length: Sequence[float], prec: int = L.F32):
         self.h = C.c_void_p()
         L.call("mfic_conv_create", ctx.h, (C.c_size_t * 3)(*n), (C.c_double * 3)(*length), prec, C.byref(self.h))
+        ctx.adopt(self)
 
     @staticmethod
     def _op(op):
@@ -164,6 +165,7 @@ class PltTables:
         self.h = C.c_void_p()
         self.ng = ng
         L.call("mfic_plt_create", ctx.h, lattice, ng, C.byref(self.h))
+        ctx.adopt(self)
 
     def download(self):
         nk = self.ng * self.ng * (self.ng // 2 + 1)
@@ -230,6 +232,7 @@ class ICRun:
         self._c = cfg.to_c()
         self.h = C.c_void_p()
         L.call("mfic_run_create", ctx.h, C.byref(self._c), C.byref(self.cosmo), C.byref(self.h))
+        ctx.adopt(self)
 
     def execute(self, blocking: bool = True) -> None:
         L.call("mfic_run_execute", self.h, int(blocking))
