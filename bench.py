@@ -11,7 +11,9 @@ timing on the library stream during the timed steps), "cpu_baseline" (the
 CPU FP64 oracle on a bounded 256^3 sample of the same configuration),
 "fp64" (same workload with FP64 bulk), "accuracy" (FP32 vs FP64 device
 fields, rel-L2 / max error per component) and "batch" (48 independent 2LPT
-boxes of 128^3-256^3 replayed as one HIP graph with autotuned tiles).
+boxes of 128^3-256^3 replayed as one HIP graph with autotuned tiles) and
+"adjoint" (the hand-written adjoint of the 2LPT map under a CIC density loss
+at 256^3: timing, FP64 and finite-difference gradient checks).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
        python bench.py --calibrate-copy          (copy kernel for counter scaling)
@@ -199,6 +201,93 @@ def batch_bench(reps=3):
     return out
 
 
+def adjoint_bench(n=256, box=500.0, reps=3, fd_cells=64):
+    """Differentiable 2LPT at 256^3, box 500 Mpc/h, z_start 24: loss =
+    sum_cells (CIC density of the displaced particles - target)^2 with the
+    target from seed 54321 through the same map, noise from seed 12345.
+    Times one FP32 loss forward + backward (hand-written adjoint of the map,
+    torch autograd for the CIC), and checks the gradient against an FP64
+    reference gradient and against FP64 central differences (eps 1e-3) on 64
+    cells drawn with mt19937_64(7)."""
+    import numpy as np
+    import torch
+
+    from monofonic_amd import F32, F64, Context, Grid, lpt
+    from monofonic_amd.adjoint import LPT2, cic_density, lpt2_map
+    from monofonic_amd.batch import MT19937_64
+    ctx = Context(0)
+
+    def real_noise(seed):
+        g = Grid(ctx, (n,) * 3, (box,) * 3, prec=F64)
+        lpt.ngenic_fill(g, seed)
+        g.FourierTransformBackward()
+        x = g.download_r().copy()
+        g.close()
+        return torch.tensor(x, device="cuda")
+
+    d64 = real_noise(12345)
+    plans = {F32: LPT2(ctx, n, box, prec=F32), F64: LPT2(ctx, n, box, prec=F64)}
+    with torch.no_grad():
+        tgt64 = cic_density(plans[F64].forward(real_noise(54321)), box)
+    tgt32 = tgt64.float()
+
+    def loss_of(d, prec):
+        tgt = tgt64 if prec == F64 else tgt32
+        return ((cic_density(lpt2_map(d, plans[prec]), box) - tgt) ** 2).sum()
+
+    def grad_of(prec):
+        d = (d64 if prec == F64 else d64.float()).clone().requires_grad_(True)
+        loss = loss_of(d, prec)
+        loss.backward()
+        return loss.item(), d.grad.double()
+
+    grad_of(F32)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        loss32, g32 = grad_of(F32)
+    torch.cuda.synchronize()
+    ms_loss = (time.perf_counter() - t) / reps * 1e3
+    x32 = d64.float()
+    cot = torch.ones((3, n, n, n), dtype=torch.float32, device="cuda")
+    plans[F32].forward(x32)
+    plans[F32].adjoint(x32, cot)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        plans[F32].forward(x32)
+    torch.cuda.synchronize()
+    ms_fwd = (time.perf_counter() - t) / reps * 1e3
+    t = time.perf_counter()
+    for _ in range(reps):
+        plans[F32].adjoint(x32, cot)
+    torch.cuda.synchronize()
+    ms_adj = (time.perf_counter() - t) / reps * 1e3
+    loss64, g64 = grad_of(F64)
+    rel32 = (torch.linalg.norm(g32 - g64) / torch.linalg.norm(g64)).item()
+    g = MT19937_64(7)
+    idx = [g() % n ** 3 for _ in range(fd_cells)]
+    rms = g64.square().mean().sqrt().item()
+    eps = 1e-3
+    errs = []
+    with torch.no_grad():
+        for c in idx:
+            e = torch.zeros(n ** 3, dtype=torch.float64, device="cuda")
+            e[c] = eps
+            e = e.reshape(n, n, n)
+            fd = (loss_of(d64 + e, F64) - loss_of(d64 - e, F64)).item() / (2 * eps)
+            errs.append(abs(fd - g64.reshape(-1)[c].item()) / rms)
+    for p in plans.values():
+        p.close()
+    ctx.close()
+    return {"workload": f"2LPT {n}^3 box {box:g} Mpc/h z_start 24, CIC loss vs target (seed 54321), noise seed 12345",
+            "fp32_loss_forward_backward_ms": round(ms_loss, 3), "map_forward_ms": round(ms_fwd, 3),
+            "map_adjoint_ms": round(ms_adj, 3), "map_cells_per_s": round(n ** 3 / (ms_fwd * 1e-3), 1),
+            "loss_fp32": loss32, "loss_fp64": loss64, "grad_fp32_vs_fp64_rel_l2": rel32,
+            "fd_cells": fd_cells, "fd_eps": eps, "fd_max_err_over_rms_grad": max(errs),
+            "fd_median_err_over_rms_grad": float(np.median(errs))}
+
+
 def accuracy(ctx, n, box):
     """FP32 vs FP64 device component fields (same seed): rel-L2 and max error"""
     import numpy as np
@@ -294,6 +383,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box of this workload fits one "
@@ -341,6 +431,9 @@ def main():
     if not args.no_batch:
         log("batched boxes: 48 x 2LPT FP32 as one HIP graph")
         line["batch"] = batch_bench()
+    if not args.no_adjoint:
+        log("differentiable 2LPT: FP32 loss gradient at 256^3, FP64 and finite-difference checks")
+        line["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         log("CPU baseline (oracle) on a 256^3 sample")
         line["cpu_baseline"] = cpu_baseline()

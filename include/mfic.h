@@ -288,6 +288,30 @@ int mfic_graph_destroy(mfic_graph g);
  * time of one execution. */
 int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double *best_ms);
 
+/* --------------------------------------------------- differentiable 2LPT --- */
+/* No reference counterpart (monofonIC has no derivative of its IC map).
+ * Forward map from a real-space white-noise grid delta to the 2LPT
+ * displacement grids, with the NGenIC convention of empty Nyquist planes:
+ *   w = P_N r2c(delta); phi1 = -w A(k) / (k^2 volfac) (ic_generator.cc:378-383,
+ *   509-516); phi2 = lap^-1 of the Orszag 2LPT source (ic_generator.cc:
+ *   523-555); Psi_d = c2r(i k_d (g1 phi1 + g2 phi2)) * unit, DC := 0
+ *   (ic_generator.cc:931-970); unit = 1 gives Mpc/h.
+ * mfic_lpt2_adjoint maps cotangents of Psi_x, Psi_y, Psi_z back to the
+ * gradient with respect to delta (reverse mode, hand-written), recomputing
+ * phi1 from delta instead of storing forward intermediates.  All grids are
+ * real-space n^3 grids of the plan's precision. */
+typedef struct mfic_lpt2_s *mfic_lpt2;
+int mfic_lpt2_create(mfic_ctx ctx, size_t n, double L, int prec, const mfic_cosmo_params *cosmo, double unit,
+                     mfic_lpt2 *out);
+int mfic_lpt2_destroy(mfic_lpt2 p);
+int mfic_lpt2_forward(mfic_lpt2 p, mfic_grid noise, mfic_grid psi_x, mfic_grid psi_y, mfic_grid psi_z);
+int mfic_lpt2_adjoint(mfic_lpt2 p, mfic_grid noise, mfic_grid cot_x, mfic_grid cot_y, mfic_grid cot_z,
+                      mfic_grid grad);
+/* a grid over caller-owned device memory in the layout above (for example a
+ * framework tensor of shape [n0][n1][n2+2]); destroying it frees nothing */
+int mfic_grid_wrap(mfic_ctx ctx, const size_t n[3], const double L[3], int prec, int space, void *device_ptr,
+                   mfic_grid *out);
+
 #ifdef __cplusplus
 }
 #endif

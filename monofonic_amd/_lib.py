@@ -123,6 +123,13 @@ SIGNATURES = {
     "mfic_graph_launch": (C.c_int, [_P, C.c_int]),
     "mfic_graph_destroy": (C.c_int, [_P]),
     "mfic_autotune": (C.c_int, [_P, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mfic_lpt2_create": (C.c_int, [_P, C.c_size_t, C.c_double, C.c_int, C.POINTER(CosmoParams), C.c_double,
+                                   C.POINTER(_P)]),
+    "mfic_lpt2_destroy": (C.c_int, [_P]),
+    "mfic_lpt2_forward": (C.c_int, [_P] * 5),
+    "mfic_lpt2_adjoint": (C.c_int, [_P] * 6),
+    "mfic_grid_wrap": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_double), C.c_int, C.c_int, _P,
+                                 C.POINTER(_P)]),
 }
 
 _lib = None
