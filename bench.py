@@ -384,6 +384,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box of this workload fits one "
@@ -395,6 +396,10 @@ def main():
 
     n, box = args.n, 1000.0 * args.n / 512
     ctx = Context(0)
+    if not args.no_tune:
+        from monofonic_amd.batch import autotune
+        log(f"autotune tiles for {n}^3 3LPT FP32 (plan time)")
+        tuned_ms = autotune(ctx, n, F32, 3)
     log(f"plan: 3LPT+PLT bcc n={n} box={box} FP32")
     cfg = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=BCC, plt=True, prec=F32, seed=12345,
                    particles=True)
@@ -411,6 +416,7 @@ def main():
         "config": {"workload": f"3LPT+PLT bcc {n}^3 ({3 * n // 2}^3 padded), box {box:g} Mpc/h, z_start 24, "
                                "FP32 bulk, FP64 q+Psi", "n": n, "padded": 3 * n // 2, "box": box, "zstart": 24.0,
                    "lpt_order": 3, "lattice": "bcc", "plt": True, "particles": 2 * n ** 3},
+        "autotune": None if args.no_tune else {"tuned_3lpt_fields_ms": round(tuned_ms, 3)},
         "roofline": roofline(rep, args.steps),
         "kernels": kernel_table(rep, args.steps),
     }

@@ -85,15 +85,26 @@ template <typename T> struct LdOperand {
     }
     __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
         const int i = src_of(pos, n);
-        if (i < 0) return cx<T>{0, 0};
-        const cx<T> v = c.p[(long)i * n * kc];
+        return i < 0 ? cx<T>{0, 0} : c.p[(long)i * n * kc];
+    }
+    __device__ __forceinline__ cx<T> fix(const Col& c, int pos, cx<T> v) const {
+        const int i = src_of(pos, n);
+        if (i < 0) return v;
         const double g = gradc(i, n, kf);
         const double s = (c.al * g + c.be) * g + c.ga;
         return c.imag ? cx<T>{(T)(-s * (double)v.y), (T)(s * (double)v.x)}
                       : cx<T>{(T)(s * (double)v.x), (T)(s * (double)v.y)};
     }
 };
-// gradient factor of index idx in the working precision (0 at Nyquist)
+// gradient factor of the source index behind padded position pos, in the
+// working precision: source index pos (pos <= n/2) or pos - n/2 (pos > n),
+// signed pos or pos - 3n/2, 0 at the source Nyquist n/2 (the zero band
+// n/2 < pos <= n carries no data, its factor is never used)
+template <typename T> __device__ __forceinline__ T grad_pad(int pos, int n, T kf) {
+    const int s = pos - (pos > n / 2 ? 3 * n / 2 : 0);
+    return pos == n / 2 ? T(0) : (T)s * kf;
+}
+// the same for an unpadded index
 template <typename T> __device__ __forceinline__ T grad_t(int idx, int n, T kf) {
     return idx == n / 2 ? T(0) : (T)(idx - (idx > n / 2) * n) * kf;
 }
@@ -107,10 +118,11 @@ template <typename T> struct LdPow {
     __device__ __forceinline__ Col column(long col) const { return Col{f + col}; }
     __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
         const int i = src_of(pos, n);
-        if (i < 0) return cx<T>{0, 0};
-        const cx<T> v = c.p[(long)i * n * kc];
-        const T g = grad_t<T>(i, n, kf);
-        const T m = p == 0 ? scale : (p == 1 ? scale * g : scale * (g * g));
+        return i < 0 ? cx<T>{0, 0} : c.p[(long)i * n * kc];
+    }
+    __device__ __forceinline__ cx<T> fix(const Col&, int pos, cx<T> v) const {
+        const T g = grad_pad<T>(pos, n, kf);
+        const T m = scale * (p > 0 ? g : T(1)) * (p > 1 ? g : T(1));
         return cx<T>{m * v.x, m * v.y};
     }
 };
@@ -128,11 +140,11 @@ template <typename T> struct LdAx1Mul {
         return Col{kb + ip * np * kc + k, r == 0 ? sign : (r == 1 ? sign * gk : sign * (gk * gk))};
     }
     __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
-        const int j = src_of(pos, n);
-        if (j < 0) return cx<T>{0, 0};
-        const cx<T> v = c.p[(long)pos * kc];
-        const T g = grad_t<T>(j, n, kf);
-        const T m = q == 0 ? c.m : (q == 1 ? c.m * g : c.m * (g * g));
+        return src_of(pos, n) < 0 ? cx<T>{0, 0} : c.p[(long)pos * kc];
+    }
+    __device__ __forceinline__ cx<T> fix(const Col& c, int pos, cx<T> v) const {
+        const T g = grad_pad<T>(pos, n, kf);
+        const T m = c.m * (q > 0 ? g : T(1)) * (q > 1 ? g : T(1));
         return cx<T>{m * v.x, m * v.y};
     }
 };
@@ -160,6 +172,7 @@ template <typename T> struct LdAx1 {
         if (skip_band && pos > n / 2 && pos <= n) return cx<T>{0, 0};
         return c.p[(long)pos * kc];
     }
+    __device__ __forceinline__ cx<T> fix(const Col&, int, cx<T> v) const { return v; }
 };
 template <typename T> struct StAx1 {
     cx<T>* kb;
@@ -184,6 +197,7 @@ template <typename T> struct LdPadCol {
         return Col{kb + (long)pad_of(j, n) * kc + k};
     }
     __device__ __forceinline__ cx<T> load(const Col& c, int pos) const { return c.p[(long)pos * np * kc]; }
+    __device__ __forceinline__ cx<T> fix(const Col&, int, cx<T> v) const { return v; }
 };
 // pass A' store: unpad rules + output operator (+ optional -1/k^2) into the N-grid
 template <typename T> struct StUnpad {
@@ -237,8 +251,11 @@ k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp) {
     const long c = active ? col : 0;
     const auto lc = ld.column(c);
     cx<T> v[E];
+    // every load of the pencil is issued before any multiplier is applied
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? ld.load(lc, q + P * e) : cx<T>{0, 0};
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = ld.fix(lc, q + P * e, v[e]);
     fft_regs<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
         const auto sc = st.column(c);
@@ -600,12 +617,12 @@ static void hessians_to_kbuf_t(mfic_ctx ctx, const void* f, size_t n, double L, 
         for (int p = 2; p >= 0; --p) {                                                                         \
             LdPow<T> la{static_cast<const cx<T>*>(f), N, kc, p, (T)kf, (T)rfac};                               \
             StPadCol<T> sa{xb + p * stride, N, np, kc};                                                        \
-            launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs);             \
+            launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rAh", (double)colsA * (N + np) * cs);            \
             for (int h = 0; h < 6; ++h) {                                                                      \
                 if (PQR[h][0] != p) continue;                                                                  \
                 LdAx1Mul<T> lb{xb + p * stride, N, np, kc, PQR[h][1], PQR[h][2], (T)kf, (T)-1};                \
                 StAx1<T> sb{ob + h * ostride, N, np, kc, 0};                                                   \
-                launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs);         \
+                launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rBh", (double)colsB * (N + np) * cs);        \
             }                                                                                                  \
         }                                                                                                      \
         break;
