@@ -157,7 +157,7 @@ MFIC_PLAN(144, 12, 2, 12, 12, 0, 0)
 MFIC_PLAN(192, 12, 3, 4, 4, 12, 0)
 MFIC_PLAN(256, 16, 2, 16, 16, 0, 0)
 MFIC_PLAN(288, 12, 3, 2, 12, 12, 0)
-MFIC_PLAN(384, 24, 3, 8, 12, 4, 0)
+MFIC_PLAN(384, 12, 4, 12, 4, 4, 2)
 MFIC_PLAN(512, 16, 3, 16, 8, 4, 0)
 MFIC_PLAN(576, 24, 3, 8, 6, 12, 0)
 MFIC_PLAN(768, 24, 3, 12, 8, 8, 0)

@@ -70,8 +70,8 @@ template <int M, int NI> constexpr int fused_rows_per_block() {
     return tpr >= 256 ? 1 : (256 / tpr > 8 ? 8 : 256 / tpr);
 }
 
-// FP32: ask for three waves per SIMD (at most 168 VGPRs), which the LDS of
-// a 16-input row (52 KB at 768) also allows
+// FP32: ask for at least three waves per SIMD (at most 168 VGPRs); forcing
+// four spills registers in the 16-input pass and measured slower
 template <typename T, int M, class PR>
 __global__ void __launch_bounds__((fused_rows_per_block<M, PR::NI>() * PR::NI * lane_slot<Plan<M>::P>()))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8)))
