@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "cosmology.hpp"
+#include "fused.hpp"
 #include "kspace.hpp"
 #include "lpt.hpp"
 #include "mfic_internal.hpp"
@@ -353,18 +354,25 @@ int mfic_run_execute(mfic_run r, int blocking) {
         for (int q = 0; q < 6 * nsub; ++q) outs[q] = og[q]->d;
         assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3, outs,
                      nsub);
-        for (int q = 0; q < 6 * nsub; ++q) {
-            if (c.prec == MFIC_F64) fft_c2r<double>(r->ctx, (double*)og[q]->d, nn, norm);
-            else fft_c2r<float>(r->ctx, (float*)og[q]->d, nn, norm);
-            og[q]->space = MFIC_RSPACE;
-        }
         if (c.particles) {
+            // strided passes per component, then the contiguous pass fused
+            // with the readout of the three components of each sub-lattice
+            for (int q = 0; q < 6 * nsub; ++q) {
+                if (c.prec == MFIC_F64) fft_c2r_cols<double>(r->ctx, (double*)og[q]->d, nn);
+                else fft_c2r_cols<float>(r->ctx, (float*)og[q]->d, nn);
+            }
             for (int s = 0; s < nsub; ++s) {
                 const void* fp[3] = {og[6 * s]->d, og[6 * s + 1]->d, og[6 * s + 2]->d};
                 const void* fv[3] = {og[6 * s + 3]->d, og[6 * s + 4]->d, og[6 * s + 5]->d};
                 const size_t pbase = (size_t)s * c.n * c.n * c.n;
-                lattice_readout3(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, 0.5 * s, pbase, r->d_pos);
-                lattice_readout3(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, 0.5 * s, pbase, r->d_vel);
+                rows_c2r_particles(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, 0.5 * s, pbase, r->d_pos);
+                rows_c2r_particles(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, 0.5 * s, pbase, r->d_vel);
+            }
+        } else {
+            for (int q = 0; q < 6; ++q) {
+                if (c.prec == MFIC_F64) fft_c2r<double>(r->ctx, (double*)og[q]->d, nn, norm);
+                else fft_c2r<float>(r->ctx, (float*)og[q]->d, nn, norm);
+                og[q]->space = MFIC_RSPACE;
             }
         }
         if (blocking) MFIC_HIP(hipStreamSynchronize(r->ctx->stream));

@@ -342,6 +342,17 @@ template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], dou
     fft_rows_c2r<T>(ctx, data, n[0] * n[1], (int)n[2], scale);
 }
 
+// the two strided c2r passes only (axis 0, axis 1): the contiguous-axis pass
+// is left to a fused row kernel
+template <typename T> void fft_c2r_cols(mfic_ctx ctx, T* data, const size_t n[3]) {
+    check_sizes(n);
+    const long nc = (long)(n[2] / 2 + 1);
+    fft_pencils<T>(ctx, data, (int)n[0], (long)n[1] * nc, 0, (int)(n[1] * nc), 1, false, 1.0, "_ax0");
+    fft_pencils<T>(ctx, data, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], false, 1.0, "_ax1");
+}
+template void fft_c2r_cols<float>(mfic_ctx, float*, const size_t*);
+template void fft_c2r_cols<double>(mfic_ctx, double*, const size_t*);
+
 template void fft_r2c<float>(mfic_ctx, float*, const size_t*, double);
 template void fft_r2c<double>(mfic_ctx, double*, const size_t*, double);
 template void fft_c2r<float>(mfic_ctx, float*, const size_t*, double);

@@ -196,6 +196,104 @@ static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, i
     MFIC_LAUNCH_CHECK();
 }
 
+// Last c2r pass of the three displacement (or velocity) components fused
+// with the lattice readout (particle_generator.hh:499-544, 597-639): each
+// group transforms one component row in LDS, then the row's particles are
+// written as interleaved FP64 [p][3], q + Psi for positions
+// ((ijk + shift)/N * lunit, grid_fft.hh:242-252) or Psi for velocities.
+template <typename T, int M>
+__global__ void __launch_bounds__((fused_rows_per_block<M, 3>() * 3 * lane_slot<Plan<M>::P>()))
+k_rows_c2r_particles(const cx<T>* __restrict__ f0, const cx<T>* __restrict__ f1, const cx<T>* __restrict__ f2,
+                     long nrows, T scale, const cx<T>* __restrict__ tw, double* __restrict__ dst, int is_pos,
+                     double lunit, double shift, size_t pbase) {
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, SL = lane_slot<P>(), N = 2 * M;
+    constexpr int RP = row_lds_pitch<M>(), RB = fused_rows_per_block<M, 3>(), TPR = 3 * SL;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int tr = threadIdx.x % TPR, rb = threadIdx.x / TPR;
+    const int g = tr / SL, q = tr % SL;
+    const long row = (long)blockIdx.x * RB + rb;
+    const bool active = row < nrows;
+    const LdsRowMap lm{(rb * 3 + g) * RP};
+    const cx<T>* uw = tw + plan_tw_size<M>();
+    if (q < P) {
+        const cx<T>* src = (g == 0 ? f0 : (g == 1 ? f1 : f2)) + (active ? row : 0) * (long)(M + 1);
+        cx<T> v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            cx<T> x = active ? src[k] : cx<T>{0, 0};
+            if (k == 0) x.y = 0;
+            lds[lm.at(k)] = x;
+        }
+        if (q == 0) lds[lm.at(M)] = active ? cx<T>{src[M].x, (T)0} : cx<T>{0, 0};
+        WaveSync::sync();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            const cx<T> xk = lds[lm.at(k)], xm = lds[lm.at(M - k)];
+            const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
+            const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
+            const cx<T> B = cmulc(D, uw[k]);
+            v[e] = cx<T>{A.x - B.y, A.y + B.x};
+        }
+        WaveSync::sync();
+        fft_regs<T, M, false, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = cscale(v[e], scale);
+    }
+    __syncthreads();
+    if (!active) return;
+    const int i = (int)(row / N), j = (int)(row % N);
+    double* out = dst + 3 * (pbase + (size_t)row * N);
+    for (int idx = tr; idx < 3 * N; idx += TPR) {
+        const int p = idx / 3, c = idx - 3 * p;
+        const cx<T> w = lds[(rb * 3 + c) * RP + (p >> 1) + ((p >> 1) >> 4)];
+        double val = (double)((p & 1) ? w.y : w.x);
+        if (is_pos) {
+            const int ijk = c == 0 ? i : (c == 1 ? j : p);
+            val = (((double)ijk + shift) / (double)N) * lunit + val;
+        }
+        out[idx] = val;
+    }
+}
+
+template <typename T, int M>
+static void launch_particles(mfic_ctx ctx, const void* const f[3], int is_pos, double lunit, double shift,
+                             size_t pbase, double* dst) {
+    constexpr int SL = lane_slot<Plan<M>::P>(), RB = fused_rows_per_block<M, 3>(), RP = row_lds_pitch<M>();
+    const long nrows = (long)(2 * M) * (2 * M);
+    const size_t lds = (size_t)RB * 3 * RP * sizeof(cx<T>);
+    MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "particle row pass exceeds the LDS of a workgroup");
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const double N = 2.0 * M;
+    ProfScope prof(ctx, std::string("rows_c2r_particles_n") + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   (double)nrows * (3.0 * (M + 1) * sizeof(cx<T>) + 3.0 * 2 * M * 8.0));
+    dim3 grid((unsigned)((nrows + RB - 1) / RB));
+    hipLaunchKernelGGL((k_rows_c2r_particles<T, M>), grid, dim3(RB * 3 * SL), lds, ctx->stream,
+                       static_cast<const cx<T>*>(f[0]), static_cast<const cx<T>*>(f[1]),
+                       static_cast<const cx<T>*>(f[2]), nrows, (T)(1.0 / std::sqrt(N * N * N)), tw, dst, is_pos,
+                       lunit, shift, pbase);
+    MFIC_LAUNCH_CHECK();
+}
+
+#define MFIC_PARTICLE_ROWS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384)
+
+void rows_c2r_particles(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit, double shift,
+                        size_t pbase, double* dst) {
+    switch (n / 2) {
+#define X(M)                                                                                  \
+    case M:                                                                                   \
+        if (prec == MFIC_F64) launch_particles<double, M>(ctx, f, is_pos, lunit, shift, pbase, dst); \
+        else launch_particles<float, M>(ctx, f, is_pos, lunit, shift, pbase, dst);            \
+        break;
+        MFIC_PARTICLE_ROWS(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported particle grid size " + std::to_string(n));
+    }
+}
+
 #define MFIC_FUSED_SIZES(X) X(12) X(18) X(24) X(36) X(48) X(72) X(96) X(144) X(192) X(288) X(384) X(576) X(768)
 
 template <typename T, class PR>

@@ -17,6 +17,11 @@ enum class FusedKind {
 // and axis-0 r2c passes
 void fused_source_rows(mfic_ctx ctx, int prec, FusedKind kind, size_t n, const void* in, void* out);
 int fused_inputs(FusedKind kind);
+// last (contiguous-axis) c2r pass of three component grids (k-space after the
+// axis-0 and axis-1 passes, [n][n][n/2+1] complex) fused with the lattice
+// readout into FP64 [p][3] at particle offset pbase
+void rows_c2r_particles(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit, double shift,
+                        size_t pbase, double* dst);
 int fused_outputs(FusedKind kind);
 
 }  // namespace mfic

@@ -114,6 +114,7 @@ template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], dou
 bool fft_length_supported(size_t n);
 
 // pass-level entry points used by fused pipelines
+template <typename T> void fft_c2r_cols(mfic_ctx ctx, T* data, const size_t n[3]);
 template <typename T> void fft_rows_r2c(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
 template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
 // complex pencils of length L: element (o, p, t) at data + o*os + p*ps + t,
