@@ -227,6 +227,16 @@ __device__ inline uint64_t sel12(const uint64_t x[12], int c) {
     return v;
 }
 
+// FP32 output fast path: y approximates the correctly rounded double result
+// within a few double ulps.  When every double within 64 ulps of y rounds
+// to the same float, that float is also the rounding of the correctly
+// rounded result, so the FP32 noise stays bit-identical.
+__device__ __forceinline__ bool float_rounding_settled(double y) {
+    const double d = fabs(y) * 0x1p-46;
+    const float f = (float)y;
+    return (float)(y - d) == f && (float)(y + d) == f;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64)
 k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, CrTables tabs) {
@@ -248,10 +258,23 @@ k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, 
         } while (ampl == 0 || ampl == 1);
         if (i == nh || j == nh || k == nh) continue;
         if (i == 0 && j == 0 && k == 0) continue;
-        ampl = sqrt(-cr_log(ampl, tabs));
-        double sn, cs;
-        cr_sincos(phase, &sn, &cs, tabs);
-        const double zr = ampl * cs, zi = ampl * sn;
+        double zr, zi;
+        bool done = false;
+        if constexpr (sizeof(T) == 4) {
+            const double a = sqrt(-log(ampl));
+            double sn, cs;
+            sincos(phase, &sn, &cs);
+            zr = a * cs;
+            zi = a * sn;
+            done = float_rounding_settled(zr) && float_rounding_settled(zi);
+        }
+        if (!done) {
+            const double a = sqrt(-cr_log(ampl, tabs));
+            double sn, cs;
+            cr_sincos(phase, &sn, &cs, tabs);
+            zr = a * cs;
+            zi = a * sn;
+        }
         if (k > 0) {
             g[((long)i * n + j) * nc + k] = cx<T>{(T)zr, (T)zi};
         } else if (i == 0) {

@@ -1,6 +1,7 @@
 """Device NGenIC white noise vs the CPU oracle (random_ngenic.cc:84-154):
 bit-exact in FP64 (RANLUX stream, rejection loop, correctly rounded
-log/sin/cos on both sides), exact rounding of the FP64 values in FP32."""
+log/sin/cos on both sides), exact rounding of the FP64 values in FP32
+(including the FP32 fast path over 512^3 modes)."""
 import numpy as np
 import pytest
 
@@ -29,6 +30,24 @@ def test_noise_fp32_is_rounded_fp64(ctx):
     ref = O.ngenic_fill(seed, n).astype(np.float32)
     assert np.array_equal(got, ref)
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,seed", [(256, 12345), (512, 12345)])
+def test_noise_fp32_fast_path_large(ctx, n, seed):
+    """the FP32 fill takes a fast path for log/sin/cos with a rounding
+    check; over 10^7-10^8 modes it must still equal the rounding of the
+    correctly rounded FP64 fill (itself bit-exact with the oracle)"""
+    print(f"ngenic FP32 vs FP64 n={n}", flush=True)
+    g32 = Grid(ctx, (n,) * 3, (100.0,) * 3, prec=F32)
+    lpt.ngenic_fill(g32, seed)
+    got = g32.download(np.float32)
+    g32.close()
+    g64 = Grid(ctx, (n,) * 3, (100.0,) * 3, prec=F64)
+    lpt.ngenic_fill(g64, seed)
+    ref = g64.download(np.float64).astype(np.float32)
+    g64.close()
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
 @pytest.mark.gpu
