@@ -25,7 +25,7 @@ namespace mfic {
 // Hessian component order in every source: xx yy zz xy xz yz
 // 2LPT source (ic_generator.cc:536-541)
 struct ProdSrc2 {
-    static constexpr int NI = 6, NO = 1;
+    static constexpr int NI = 6, NO = 1, IPG = 1, WAVES = 3;
     __device__ static __forceinline__ void eval(const double* h, double* o) {
         o[0] = h[0] * (h[1] + h[2]) + h[1] * h[2] - h[3] * h[3] - h[4] * h[4] - h[5] * h[5];
     }
@@ -34,7 +34,7 @@ struct ProdSrc2 {
 // factor: Qzz = xx yy - xy^2, Qyz = xy xz, Qxx = yz^2, Qyy = xz^2
 // (ic_generator.cc:581-585)
 struct ProdSrc2Q {
-    static constexpr int NI = 6, NO = 5;
+    static constexpr int NI = 6, NO = 5, IPG = 1, WAVES = 3;
     __device__ static __forceinline__ void eval(const double* h, double* o) {
         ProdSrc2::eval(h, o);
         o[1] = h[0] * h[1] - h[3] * h[3];
@@ -47,7 +47,9 @@ struct ProdSrc2Q {
 // ic_generator.cc:581-609) and A3 (ic_generator.cc:614-627).  Inputs: phi1
 // Hessians a[0..5], phi2 Hessians b[6..11], Qzz Qyz Qxx Qyy [12..15]
 struct ProdSrc3 {
-    static constexpr int NI = 16, NO = 4;
+    // two inputs per lane group: 4-wave workgroups; registers for two waves
+    // per SIMD (no spills), i.e. two workgroups per CU
+    static constexpr int NI = 16, NO = 4, IPG = 2, WAVES = 2;
     __device__ static __forceinline__ void eval(const double* h, double* o) {
         const double* a = h;
         const double* b = h + 6;
@@ -70,17 +72,18 @@ template <int M, int NI> constexpr int fused_rows_per_block() {
     return tpr >= 256 ? 1 : (256 / tpr > 8 ? 8 : 256 / tpr);
 }
 
-// FP32: ask for at least three waves per SIMD (at most 168 VGPRs); forcing
-// four spills registers in the 16-input pass and measured slower
+// FP32: at least PR::WAVES waves per SIMD (register budget 512 / WAVES)
 template <typename T, int M, class PR>
-__global__ void __launch_bounds__((fused_rows_per_block<M, PR::NI>() * PR::NI * lane_slot<Plan<M>::P>()))
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8)))
+__global__ void __launch_bounds__((fused_rows_per_block<M, PR::NI / PR::IPG>() * (PR::NI / PR::IPG) * lane_slot<Plan<M>::P>()))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PR::WAVES : 1, 8)))
 k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out, long ostride, int kc, long nrows,
              T scale, const cx<T>* __restrict__ tw) {
-    constexpr int NI = PR::NI, NO = PR::NO, E = Plan<M>::E, P = Plan<M>::P, SL = lane_slot<P>();
-    constexpr int RP = row_lds_pitch<M>(), RB = fused_rows_per_block<M, NI>(), TPR = NI * SL;
+    constexpr int NI = PR::NI, NO = PR::NO, IPG = PR::IPG, NG = NI / IPG;
+    constexpr int E = Plan<M>::E, P = Plan<M>::P, SL = lane_slot<P>();
+    constexpr int RP = row_lds_pitch<M>(), RB = fused_rows_per_block<M, NG>(), TPR = NG * SL;
     constexpr int NPOS = (M + TPR - 1) / TPR;
     static_assert(P <= 64 && 64 % SL == 0, "a transform must fit in one wavefront");
+    static_assert(NI % IPG == 0, "inputs per group must divide the inputs");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int tr = threadIdx.x % TPR, rb = threadIdx.x / TPR;
@@ -88,36 +91,48 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
     const bool lane = q < P;
     const long row = (long)blockIdx.x * RB + rb;
     const bool active = row < nrows;
-    const LdsRowMap lm{(rb * NI + g) * RP};
     const cx<T>* uw = tw + plan_tw_size<M>();
 
-    // 1. c2r of input g: half-spectrum row (kz < kc, kz = 0 and the padded
-    // Nyquist real) -> M complex = 2M real values, kept in the group's region
+    // 1. c2r of inputs g*IPG .. g*IPG+IPG-1: half-spectrum rows (kz < kc, kz = 0
+    // and the padded Nyquist real) -> M complex = 2M real values, each kept in
+    // its input's region; every row load is issued before the first transform
     if (lane) {
-        const cx<T>* src = in + g * istride + (active ? row : 0) * (long)kc;
+        cx<T> raw[IPG][E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int k = q + P * e;
-            cx<T> x = (active && k < kc) ? src[k] : cx<T>{0, 0};
-            if (k == 0) x.y = 0;
-            lds[lm.at(k)] = x;
+        for (int t = 0; t < IPG; ++t) {
+            const cx<T>* src = in + (g * IPG + t) * istride + (active ? row : 0) * (long)kc;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = q + P * e;
+                raw[t][e] = (active && k < kc) ? src[k] : cx<T>{0, 0};
+            }
         }
-        if (q == 0) lds[lm.at(M)] = cx<T>{0, 0};
-        WaveSync::sync();
-        cx<T> v[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int k = q + P * e;
-            const cx<T> xk = lds[lm.at(k)], xm = lds[lm.at(M - k)];
-            const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
-            const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
-            const cx<T> B = cmulc(D, uw[k]);
-            v[e] = cx<T>{A.x - B.y, A.y + B.x};
+        for (int t = 0; t < IPG; ++t) {
+            const LdsRowMap lm{(rb * NI + g * IPG + t) * RP};
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                cx<T> x = raw[t][e];
+                if (q + P * e == 0) x.y = 0;
+                lds[lm.at(q + P * e)] = x;
+            }
+            if (q == 0) lds[lm.at(M)] = cx<T>{0, 0};
+            WaveSync::sync();
+            cx<T> v[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = q + P * e;
+                const cx<T> xk = lds[lm.at(k)], xm = lds[lm.at(M - k)];
+                const cx<T> A = {xk.x + xm.x, xk.y - xm.y};
+                const cx<T> D = {xk.x - xm.x, xk.y + xm.y};
+                const cx<T> B = cmulc(D, uw[k]);
+                v[e] = cx<T>{A.x - B.y, A.y + B.x};
+            }
+            WaveSync::sync();
+            fft_regs<T, M, false, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+#pragma unroll
+            for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = cscale(v[e], scale);
         }
-        WaveSync::sync();
-        fft_regs<T, M, false, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
-#pragma unroll
-        for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = cscale(v[e], scale);
     }
     __syncthreads();
 
@@ -153,26 +168,29 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
     }
     __syncthreads();
 
-    // 3. r2c of output g < NO, first kc half-spectrum values written
-    if (lane && g < NO) {
-        cx<T> v[E];
+    // 3. r2c of outputs g, g + NG, ... < NO, first kc half-spectrum values written
+    if (lane) {
+        for (int o = g; o < NO; o += NG) {
+            const LdsRowMap lm{(rb * NI + o) * RP};
+            cx<T> v[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = lds[lm.at(q + P * e)];
-        WaveSync::sync();
-        fft_regs<T, M, true, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+            for (int e = 0; e < E; ++e) v[e] = lds[lm.at(q + P * e)];
+            WaveSync::sync();
+            fft_regs<T, M, true, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
 #pragma unroll
-        for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
-        WaveSync::sync();
-        if (active) {
-            cx<T>* dst = out + g * ostride + row * (long)kc;
+            for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
+            WaveSync::sync();
+            if (active) {
+                cx<T>* dst = out + o * ostride + row * (long)kc;
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int k = q + P * e;
-                if (k >= kc) continue;
-                const cx<T> zk = v[e], zm = lds[lm.at(k == 0 ? 0 : M - k)];
-                const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
-                const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
-                dst[k] = cadd(Ek, cmul(uw[k], Ok));
+                for (int e = 0; e < E; ++e) {
+                    const int k = q + P * e;
+                    if (k >= kc) continue;
+                    const cx<T> zk = v[e], zm = lds[lm.at(k == 0 ? 0 : M - k)];
+                    const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+                    const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+                    dst[k] = cadd(Ek, cmul(uw[k], Ok));
+                }
             }
         }
     }
@@ -181,8 +199,8 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
 template <typename T, int M, class PR>
 static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, int kc, long nrows, double scale,
                          const char* name) {
-    constexpr int NI = PR::NI, NO = PR::NO, SL = lane_slot<Plan<M>::P>();
-    constexpr int RB = fused_rows_per_block<M, NI>(), RP = row_lds_pitch<M>();
+    constexpr int NI = PR::NI, NO = PR::NO, NG = NI / PR::IPG, SL = lane_slot<Plan<M>::P>();
+    constexpr int RB = fused_rows_per_block<M, NG>(), RP = row_lds_pitch<M>();
     MFIC_CHECK(kc <= M, MFIC_ERR_SIZE, "half-spectrum rows longer than the padded transform");
     const size_t lds = (size_t)RB * NI * RP * sizeof(cx<T>);
     MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
@@ -190,7 +208,7 @@ static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, i
     dim3 grid((unsigned)((nrows + RB - 1) / RB));
     ProfScope prof(ctx, std::string(name) + "_n" + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    (double)nrows * (NI + NO) * kc * sizeof(cx<T>));
-    hipLaunchKernelGGL((k_rows_fused<T, M, PR>), grid, dim3(RB * NI * SL), lds, ctx->stream,
+    hipLaunchKernelGGL((k_rows_fused<T, M, PR>), grid, dim3(RB * NG * SL), lds, ctx->stream,
                        static_cast<const cx<T>*>(in), stride, static_cast<cx<T>*>(out), stride, kc, nrows, (T)scale,
                        tw);
     MFIC_LAUNCH_CHECK();
