@@ -22,14 +22,18 @@ namespace mfic {
 template <typename T, int L, int TK, bool FWD>
 __global__ void __launch_bounds__(Plan<L>::P * TK)
 k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
-          const cx<T>* __restrict__ tw, int tkp) {
+          const cx<T>* __restrict__ tw, int tkp, int gx, long ntiles, int xcd) {
     constexpr int E = Plan<L>::E, P = Plan<L>::P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int t = threadIdx.x % TK, q = threadIdx.x / TK;
-    const int tg = blockIdx.x * TK + t;
-    const bool active = tg < tlen;
-    cx<T>* base = data + (long)blockIdx.y * os + tg;
+    // one-dimensional grid of gx tiles per pencil group (XCD-aware order for
+    // the narrow groups of the axis-1 pass, whose tiles share cache lines)
+    const long tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : (long)blockIdx.x;
+    const long outer = tile / gx;
+    const int tg = (int)(tile - outer * gx) * TK + t;
+    const bool active = tg < tlen && tile < ntiles;
+    cx<T>* base = data + (active ? outer : 0) * os + tg;
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? base[(long)(q + P * e) * ps] : cx<T>{0, 0};
@@ -228,8 +232,10 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
     while (tk > 1 && P * tk > 1024) tk /= 2;
     const int tkp = tk + pad;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
-    MFIC_CHECK(nouter <= 65535, MFIC_ERR_SIZE, "too many pencil groups");
-    dim3 grid((tlen + tk - 1) / tk, (unsigned)nouter);
+    const int gx = (tlen + tk - 1) / tk;
+    const long ntiles = (long)gx * nouter;
+    const bool xcd = nouter > 1;
+    dim3 grid((unsigned)(xcd ? xcd_grid(ntiles) : ntiles));
     size_t lds = (size_t)L * tkp * sizeof(cx<T>);
     ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + (FWD ? "_fwd" : "_inv") + tag + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    2.0 * (double)tlen * L * nouter * sizeof(cx<T>));
@@ -238,7 +244,7 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
     case K:                                                                                                  \
         if constexpr (P * K <= 1024)                                                                         \
             hipLaunchKernelGGL((k_pencils<T, L, K, FWD>), grid, dim3(P * K), lds, ctx->stream, data, ps, os, \
-                               tlen, scale, tw, tkp);                                                        \
+                               tlen, scale, tw, tkp, gx, ntiles, xcd ? 1 : 0);                               \
         break;
         TKCASE(1) TKCASE(2) TKCASE(4) TKCASE(8) TKCASE(16) TKCASE(32) TKCASE(64)
 #undef TKCASE

@@ -181,6 +181,14 @@ struct LdsMap {
     __device__ __forceinline__ int at(int pos) const { return base + pos * stride; }
 };
 
+// XCD-aware tile order: workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2), so workgroup b runs on XCD b % 8.  Giving
+// each XCD a contiguous range of tiles keeps neighbouring tiles, which share
+// cache lines at their edges, in one L2.  nb must be a multiple of 8; tiles
+// past the real count come out inactive in the kernels.
+__device__ __forceinline__ long xcd_tile(long b, long nb) { return (b % 8) * (nb / 8) + b / 8; }
+__host__ __device__ constexpr long xcd_grid(long nb) { return (nb + 7) / 8 * 8; }
+
 // LDS addressing for a transform whose threads all work on one row: one pad
 // slot every 16 elements spreads the stride-R butterfly writes of the early
 // Stockham stages over the LDS banks

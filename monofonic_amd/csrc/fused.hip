@@ -89,7 +89,7 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
     const int tr = threadIdx.x % TPR, rb = threadIdx.x / TPR;
     const int g = tr / SL, q = tr % SL;
     const bool lane = q < P;
-    const long row = (long)blockIdx.x * RB + rb;
+    const long row = xcd_tile(blockIdx.x, gridDim.x) * RB + rb;
     const bool active = row < nrows;
     const cx<T>* uw = tw + plan_tw_size<M>();
 
@@ -205,7 +205,7 @@ static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, i
     const size_t lds = (size_t)RB * NI * RP * sizeof(cx<T>);
     MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
-    dim3 grid((unsigned)((nrows + RB - 1) / RB));
+    dim3 grid((unsigned)xcd_grid((nrows + RB - 1) / RB));
     ProfScope prof(ctx, std::string(name) + "_n" + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    (double)nrows * (NI + NO) * kc * sizeof(cx<T>));
     hipLaunchKernelGGL((k_rows_fused<T, M, PR>), grid, dim3(RB * NG * SL), lds, ctx->stream,

@@ -241,12 +241,12 @@ template <typename T> struct StUnpad {
 
 template <typename T, int L, int TK, bool FWD, class LD, class ST>
 __global__ void __launch_bounds__(Plan<L>::P * TK)
-k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp) {
+k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int xcd) {
     constexpr int E = Plan<L>::E, P = Plan<L>::P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int t = threadIdx.x % TK, q = threadIdx.x / TK;
-    const long col = (long)blockIdx.x * TK + t;
+    const long col = (xcd ? xcd_tile(blockIdx.x, gridDim.x) : (long)blockIdx.x) * TK + t;
     const bool active = col < ncols;
     const long c = active ? col : 0;
     const auto lc = ld.column(c);
@@ -389,19 +389,23 @@ static int pick_tile(mfic_ctx ctx) {
 }
 
 template <typename T, int L, bool FWD, class LD, class ST>
-static void launch_f(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const char* name, double bytes) {
+static void launch_f(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
+                     bool xcd = false) {
     constexpr int P = Plan<L>::P;
     const int tk = pick_tile<T, L>(ctx);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     const size_t lds = (size_t)L * tk * sizeof(cx<T>);
-    dim3 grid((unsigned)((ncols + tk - 1) / tk));
+    // axis-1 passes: 16-column tiles of rows with an odd pitch share cache
+    // lines with their neighbours, which the XCD-aware order keeps in one L2
+    const long ntiles = (ncols + tk - 1) / tk;
+    dim3 grid((unsigned)(xcd ? xcd_grid(ntiles) : ntiles));
     ProfScope prof(ctx, std::string(name) + "_L" + std::to_string(L) + (sizeof(T) == 8 ? "_f64" : "_f32"), bytes);
     switch (tk) {
 #define TKC(K)                                                                                                     \
     case K:                                                                                                        \
         if constexpr (P * K <= 1024)                                                                               \
             hipLaunchKernelGGL((k_pencils_f<T, L, K, FWD, LD, ST>), grid, dim3(P * K), lds, ctx->stream, ld, st, ncols, \
-                               tw, K);                                                                             \
+                               tw, K, xcd ? 1 : 0);                                                                \
         break;
         TKC(1) TKC(2) TKC(4) TKC(8) TKC(16) TKC(32) TKC(64)
 #undef TKC
@@ -491,7 +495,7 @@ static void c2r_from_operand_t(mfic_ctx ctx, const OperandDesc& o, size_t n, dou
 #define X(S)                                                                                                 \
     case S:                                                                                                  \
         launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs);                    \
-        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs);                    \
+        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs, true);                    \
         rows_c2r_M<T, S / 2>(ctx, static_cast<const cx<T>*>(kbuf), kc, static_cast<T*>(rout), np, (long)np * np, \
                              (T)norm);                                                                       \
         break;
@@ -537,7 +541,7 @@ static void r2c_products_t(mfic_ctx ctx, const std::vector<ProdTerm>& terms, siz
 #define X(S)                                                                                                        \
     case S:                                                                                                         \
         rows_r2c_M<T, S / 2>(ctx, pr, np, static_cast<cx<T>*>(kbuf), kc, (long)np * np, (T)1);                      \
-        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs);                            \
+        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs, true);                            \
         launch_f<T, S, true>(ctx, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
         break;
         MFIC_PADDED_SIZES(X)
@@ -562,7 +566,7 @@ static void c2r_to_kbuf_t(mfic_ctx ctx, const OperandDesc& o, size_t n, double L
 #define X(S)                                                                                   \
     case S:                                                                                    \
         launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs); \
-        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs); \
+        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs, true); \
         break;
         MFIC_PADDED_SIZES(X)
 #undef X
@@ -586,7 +590,7 @@ static void r2c_from_kbuf_t(mfic_ctx ctx, size_t n, double L, void* kbuf, void* 
     switch (np) {
 #define X(S)                                                                                                        \
     case S:                                                                                                         \
-        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs);                            \
+        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs, true);                            \
         launch_f<T, S, true>(ctx, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
         break;
         MFIC_PADDED_SIZES(X)
@@ -622,7 +626,7 @@ static void hessians_to_kbuf_t(mfic_ctx ctx, const void* f, size_t n, double L, 
                 if (PQR[h][0] != p) continue;                                                                  \
                 LdAx1Mul<T> lb{xb + p * stride, N, np, kc, PQR[h][1], PQR[h][2], (T)kf, (T)-1};                \
                 StAx1<T> sb{ob + h * ostride, N, np, kc, 0};                                                   \
-                launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rBh", (double)colsB * (N + np) * cs);        \
+                launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rBh", (double)colsB * (N + np) * cs, true);        \
             }                                                                                                  \
         }                                                                                                      \
         break;
