@@ -269,26 +269,27 @@ void lpt_sources(mfic_ctx ctx, int order, mfic_grid phi1, mfic_grid phi2, mfic_g
     // row pass (phi1 Hessians, phi2 Hessians, Q) and its outputs
     const size_t kb = C.kbuf_reals() * C.esize();
     const FusedKind k1 = order >= 3 ? FusedKind::Src2Q : FusedKind::Src2;
-    char* kin = static_cast<char*>(ctx_scratch(ctx, "fused_in", (size_t)(order >= 3 ? 16 : 6) * kb));
-    char* kout = static_cast<char*>(ctx_scratch(ctx, "fused_out", (size_t)fused_outputs(k1) * kb));
+    // inputs of the row passes, consecutive: phi1 Hessians [0, 6), phi2
+    // Hessians [6, 12), Q [12, 16).  The 2LPT pass writes its outputs (phi2
+    // source, Q) to [11, 16): slot 11 is consumed before the phi2 Hessians
+    // fill it.
+    char* kin = static_cast<char*>(ctx_scratch(ctx, "fused_in", (size_t)(order >= 3 ? 16 : 7) * kb));
     void* xbuf = ctx_scratch(ctx, "hess_x", 3 * kb);
     padded_hessians_to_kbuf(ctx, prec, phi1->d, n, L, xbuf, kin);
+    char* src2 = kin + (order >= 3 ? 11 : 6) * kb;
     // phi2 source (ic_generator.cc:536-541) and, for 3LPT, the inner 3a
     // products Qzz Qyz Qxx Qyy (ic_generator.cc:581-585) from one row pass
-    fused_source_rows(ctx, prec, k1, n, kin, kout);
-    padded_r2c_from_kbuf(ctx, prec, n, L, kout, phi2->d, MFIC_OP_ASSIGN, 0.0, true);  // -1/k^2: ic_generator.cc:554
+    fused_source_rows(ctx, prec, k1, n, kin, src2);
+    padded_r2c_from_kbuf(ctx, prec, n, L, src2, phi2->d, MFIC_OP_ASSIGN, 0.0, true);  // -1/k^2: ic_generator.cc:554
     phi2->space = MFIC_KSPACE;
     if (order < 3) return;
     // Q: truncated once (first stage of trunc(trunc(a b) c), convolution.hh:
-    // 525-532) and padded again for the outer products
-    void* qk = ctx_scratch(ctx, "qk", C.grid_reals() * C.esize());
-    for (int q = 0; q < 4; ++q) {
-        padded_r2c_from_kbuf(ctx, prec, n, L, kout + (1 + q) * kb, qk, MFIC_OP_ASSIGN, 0.0, false);
-        padded_c2r_to_kbuf(ctx, prec, OperandDesc{qk, MFIC_OPND_IDENT, {0, 0, 0, 0}}, n, L, kin + (12 + q) * kb);
-    }
+    // 525-532) and padded again for the outer products, in place
+    for (int q = 0; q < 4; ++q) padded_truncate_repad(ctx, prec, n, kin + (12 + q) * kb);
     padded_hessians_to_kbuf(ctx, prec, phi2->d, n, L, xbuf, kin + 6 * kb);
     // phi3 source (3a outer + 3b, ic_generator.cc:581-609) and A3
     // (ic_generator.cc:614-627), then -1/k^2 each
+    char* kout = static_cast<char*>(ctx_scratch(ctx, "fused_out", 4 * kb));
     fused_source_rows(ctx, prec, FusedKind::Src3, n, kin, kout);
     mfic_grid outs[4] = {phi3, A3x, A3y, A3z};
     for (int o = 0; o < 4; ++o) {

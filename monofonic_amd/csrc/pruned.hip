@@ -264,6 +264,59 @@ k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int
     }
 }
 
+// Orszag truncation followed by re-padding along one axis of a compact
+// padded buffer that is real-space along that axis: forward transform, keep
+// the source modes (not the zero band, not the source Nyquist N/2), inverse
+// transform.  Equal to the unpad (convolution.hh:600-651) then pad_insert
+// (convolution.hh:552-594) pair of a two-stage product restricted to that
+// axis; the column mask zeroes kz = N/2 and `scale` carries the pad factors.
+template <typename T, int L, int TK, class LD, class ST>
+__global__ void __launch_bounds__(Plan<L>::P * TK)
+k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int xcd, int n, int kc,
+                 int zero_kz_nyquist, T scale) {
+    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int t = threadIdx.x % TK, q = threadIdx.x / TK;
+    const long col = (xcd ? xcd_tile(blockIdx.x, gridDim.x) : (long)blockIdx.x) * TK + t;
+    const bool active = col < ncols;
+    const long c = active ? col : 0;
+    const auto lc = ld.column(c);
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = active ? ld.load(lc, q + P * e) : cx<T>{0, 0};
+    fft_regs<T, L, true>(v, q, lds, LdsMap{t, tkp}, tw);
+    const bool colzero = zero_kz_nyquist && (int)(c % kc) == n / 2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int m = q + P * e;
+        const bool keep = !colzero && (m < n / 2 || m > n);
+        v[e] = keep ? cscale(v[e], scale) : cx<T>{0, 0};
+    }
+    fft_regs<T, L, false>(v, q, lds, LdsMap{t, tkp}, tw);
+    if (active) {
+        const auto sc = st.column(c);
+#pragma unroll
+        for (int e = 0; e < E; ++e) st.store(sc, q + P * e, v[e]);
+    }
+}
+
+// axis-0 pencils of the compact buffer over every row j', column = (j', kz)
+template <typename T> struct LdAx0 {
+    const cx<T>* kb;
+    int np, kc;
+    struct Col { const cx<T>* p; };
+    __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
+    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const { return c.p[(long)pos * np * kc]; }
+};
+template <typename T> struct StAx0 {
+    cx<T>* kb;
+    int np, kc;
+    struct Col { cx<T>* p; };
+    __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
+    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const { c.p[(long)pos * np * kc] = v; }
+};
+
 // compact half-spectrum rows (kc <= M+1 values, the rest zero) -> real rows
 template <typename T, int M, int RW>
 __global__ void __launch_bounds__(Plan<M>::P * RW)
@@ -641,6 +694,60 @@ void padded_hessians_to_kbuf(mfic_ctx ctx, int prec, const void* f, size_t n, do
     const long stride = (long)(3 * n / 2) * (3 * n / 2) * (n / 2 + 1);
     if (prec == MFIC_F64) hessians_to_kbuf_t<double>(ctx, f, n, L, xbuf, out, stride);
     else hessians_to_kbuf_t<float>(ctx, f, n, L, xbuf, out, stride);
+}
+
+template <typename T, int L, class LD, class ST>
+static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, bool xcd, int n, int kc, int zk,
+                          double scale, const char* name) {
+    constexpr int P = Plan<L>::P;
+    const int tk = pick_tile<T, L>(ctx);
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const size_t lds = (size_t)L * tk * sizeof(cx<T>);
+    const long ntiles = (ncols + tk - 1) / tk;
+    dim3 grid((unsigned)(xcd ? xcd_grid(ntiles) : ntiles));
+    ProfScope prof(ctx, std::string(name) + "_L" + std::to_string(L) + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   2.0 * (double)ncols * L * sizeof(cx<T>));
+    switch (tk) {
+#define TKC(K)                                                                                                  \
+    case K:                                                                                                     \
+        if constexpr (P * K <= 1024)                                                                            \
+            hipLaunchKernelGGL((k_pencils_filter<T, L, K, LD, ST>), grid, dim3(P * K), lds, ctx->stream, ld, st, \
+                               ncols, tw, K, xcd ? 1 : 0, n, kc, zk, (T)scale);                                 \
+        break;
+        TKC(1) TKC(2) TKC(4) TKC(8) TKC(16) TKC(32) TKC(64)
+#undef TKC
+    default:
+        throw Error(MFIC_ERR_SIZE, "bad tile");
+    }
+    MFIC_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void truncate_repad_t(mfic_ctx ctx, size_t n, void* kbuf) {
+    const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
+    const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
+    cx<T>* kb = static_cast<cx<T>*>(kbuf);
+    LdAx1<T> l1{kb, N, np, kc, 0};
+    StAx1<T> s1{kb, N, np, kc, 0};
+    LdAx0<T> l0{kb, np, kc};
+    StAx0<T> s0{kb, np, kc};
+    const long cols = (long)np * kc;
+    switch (np) {
+#define X(S)                                                                                       \
+    case S:                                                                                        \
+        launch_filter<T, S>(ctx, l1, s1, cols, true, N, kc, 1, norm, "fft_truncate_ax1");          \
+        launch_filter<T, S>(ctx, l0, s0, cols, false, N, kc, 0, 1.0, "fft_truncate_ax0");          \
+        break;
+        MFIC_PADDED_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported padded size " + std::to_string(np));
+    }
+}
+
+void padded_truncate_repad(mfic_ctx ctx, int prec, size_t n, void* kbuf) {
+    if (prec == MFIC_F64) truncate_repad_t<double>(ctx, n, kbuf);
+    else truncate_repad_t<float>(ctx, n, kbuf);
 }
 
 void padded_c2r_to_kbuf(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, double L, void* kbuf) {

@@ -28,6 +28,11 @@ void padded_c2r_to_kbuf(mfic_ctx ctx, int prec, const OperandDesc& o, size_t n, 
 // consecutive compact padded buffers (out) after the axis-0 and axis-1 c2r
 // passes; xbuf is scratch for three compact buffers
 void padded_hessians_to_kbuf(mfic_ctx ctx, int prec, const void* f, size_t n, double L, void* xbuf, void* out);
+// a compact padded buffer of r2c'd rows of a product (real space along axes
+// 0 and 1) -> the same buffer truncated to the source modes with Nyquist
+// planes zeroed and padded again, ready as the input of a row pass: the unpad
+// / pad_insert pair of a two-stage product, as two in-place filter passes
+void padded_truncate_repad(mfic_ctx ctx, int prec, size_t n, void* kbuf);
 // compact padded buffer of r2c'd rows -> axis-1 and axis-0 r2c passes with the
 // unpad rules, output operator and optional -1/k^2 into the N-grid; kbuf is
 // overwritten
