@@ -59,7 +59,7 @@ k_rows_r2c(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* _
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? r[q + P * e] : cx<T>{0, 0};
-    fft_regs<T, M, true>(v, q, lds, LdsRowMap{rw * RP}, tw);
+    fft_regs_row<T, M, true>(v, q, lds, rw * RP, tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
     __syncthreads();
@@ -116,7 +116,7 @@ k_rows_c2r(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* _
         v[e] = cx<T>{A.x - B.y, A.y + B.x};  // A + i B
     }
     __syncthreads();
-    fft_regs<T, M, false>(v, q, lds, LdsRowMap{rw * RP}, tw);
+    fft_regs_row<T, M, false>(v, q, lds, rw * RP, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) r[q + P * e] = cscale(v[e], scale);

@@ -157,7 +157,7 @@ MFIC_PLAN(144, 12, 2, 12, 12, 0, 0)
 MFIC_PLAN(192, 12, 3, 4, 4, 12, 0)
 MFIC_PLAN(256, 16, 2, 16, 16, 0, 0)
 MFIC_PLAN(288, 12, 3, 2, 12, 12, 0)
-MFIC_PLAN(384, 12, 4, 12, 4, 4, 2)
+MFIC_PLAN(384, 12, 4, 4, 4, 12, 2)
 MFIC_PLAN(512, 16, 3, 16, 8, 4, 0)
 MFIC_PLAN(576, 24, 3, 8, 6, 12, 0)
 MFIC_PLAN(768, 24, 3, 12, 8, 8, 0)
@@ -194,10 +194,62 @@ __host__ __device__ constexpr long xcd_grid(long nb) { return (nb + 7) / 8 * 8; 
 // Stockham stages over the LDS banks
 struct LdsRowMap {
     int base;
-    __device__ __forceinline__ int at(int pos) const { return base + pos + (pos >> 4); }
+    __device__ __forceinline__ int at(int pos) const { return base + pos; }
 };
-// elements one row of M + 1 complex values occupies under LdsRowMap
-template <int M> constexpr int row_lds_pitch() { return M + 1 + ((M + 1) >> 4); }
+// power-of-two lane slot of a transform of P threads
+template <int P> constexpr int lane_slot() { return P <= 1 ? 1 : P <= 2 ? 2 : P <= 4 ? 4 : P <= 8 ? 8 : P <= 16 ? 16 : P <= 32 ? 32 : 64; }
+// LDS elements between consecutive rows of M + 1 complex values.  Rows
+// whose transforms share a 32-lane group sit lane_slot banks (8-byte slots
+// modulo 32) apart, so contiguous accesses of neighbouring rows interleave
+// without bank conflicts.
+template <int M> constexpr int row_lds_pitch() {
+    constexpr int SL = lane_slot<Plan<M>::P>();
+    return SL >= 32 ? M + 1 : M + 1 + (((SL - (M + 1)) % 32 + 32) % 32);
+}
+
+// Per-stage LDS swizzle of a row transform's Stockham exchanges: output x of
+// stage S is stored at base + swz(x), a bijection of [0, L) that flips the
+// low `a` bits of x by a multiple of its high bits,
+//   swz(x) = x ^ (((x >> a) * k1 ^ (x >> b) * k2) & (2^a - 1)),
+// chosen per plan so that every strided write and contiguous read of the
+// stage hits distinct 8-byte bank slots within each 16- and 32-lane group
+// (exhaustive search over a, b, k1, k2 against the access pattern of the
+// plan).  kind 0: identity; kind 2: rotation instead of xor.
+template <int L, int S> struct RowSwz { static constexpr int kind = 0, a = 0, k1 = 0, b = 0, k2 = 0; };
+#define MFIC_SWZ(L_, S_, KIND, A, K1, B, K2) \
+    template <> struct RowSwz<L_, S_> { static constexpr int kind = KIND, a = A, k1 = K1, b = B, k2 = K2; };
+MFIC_SWZ(24, 0, 1, 1, 1, 1, 0)
+MFIC_SWZ(32, 0, 1, 2, 1, 2, 0)
+MFIC_SWZ(36, 0, 1, 1, 0, 3, 1)
+MFIC_SWZ(48, 0, 1, 2, 1, 2, 0)
+MFIC_SWZ(64, 0, 1, 3, 1, 3, 0)
+MFIC_SWZ(72, 0, 1, 1, 0, 3, 1)
+MFIC_SWZ(96, 0, 1, 2, 0, 3, 1)
+MFIC_SWZ(96, 1, 1, 3, 0, 4, 4)
+MFIC_SWZ(128, 0, 1, 3, 1, 3, 0)
+MFIC_SWZ(144, 0, 1, 2, 0, 4, 1)
+MFIC_SWZ(192, 0, 1, 2, 0, 4, 1)
+MFIC_SWZ(192, 1, 1, 4, 4, 4, 0)
+MFIC_SWZ(256, 0, 1, 4, 1, 4, 0)
+MFIC_SWZ(288, 0, 1, 2, 0, 5, 1)
+MFIC_SWZ(288, 1, 1, 3, 0, 4, 7)
+MFIC_SWZ(384, 0, 1, 3, 1, 5, 1)
+MFIC_SWZ(384, 1, 1, 4, 4, 5, 2)
+MFIC_SWZ(384, 2, 1, 5, 0, 6, 6)
+MFIC_SWZ(512, 0, 1, 4, 1, 5, 1)
+MFIC_SWZ(512, 1, 1, 5, 0, 6, 8)
+#undef MFIC_SWZ
+template <int L, int S> struct LdsSwzMap {
+    int base;
+    __device__ __forceinline__ int at(int x) const {
+        using W = RowSwz<L, S>;
+        static_assert(W::kind == 0 || L % (1 << W::a) == 0, "swizzle must stay inside the row");
+        constexpr int A = (1 << W::a) - 1;
+        if constexpr (W::kind == 0) return base + x;
+        else if constexpr (W::kind == 1) return base + (x ^ ((((x >> W::a) * W::k1) ^ ((x >> W::b) * W::k2)) & A));
+        else return base + ((x & ~A) | ((x + (x >> W::a) * W::k1 + (x >> W::b) * W::k2) & A));
+    }
+};
 
 // LDS exchange synchronisation between Stockham stages
 struct BlockSync {
@@ -261,6 +313,17 @@ __device__ __forceinline__ void fft_regs(cx<T>* v, int q, cx<T>* lds, MAP m, con
     if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
     if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
     if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
+}
+
+// full transform of a row held by one thread group, exchanges through the
+// per-stage swizzled LDS slots starting at `base`
+template <typename T, int L, bool FWD, class SYNC = BlockSync>
+__device__ __forceinline__ void fft_regs_row(cx<T>* v, int q, cx<T>* lds, int base, const cx<T>* __restrict__ tw) {
+    constexpr int S = Plan<L>::S;
+    fft_stage<T, L, 0, FWD, LdsSwzMap<L, 0>, SYNC>(v, q, lds, LdsSwzMap<L, 0>{base}, tw);
+    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, LdsSwzMap<L, 1 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 1 % S>{base}, tw);
+    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, LdsSwzMap<L, 2 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 2 % S>{base}, tw);
+    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, LdsSwzMap<L, 3 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 3 % S>{base}, tw);
 }
 
 }  // namespace mfic

@@ -66,7 +66,6 @@ struct ProdSrc3 {
     }
 };
 
-template <int P> constexpr int lane_slot() { return P <= 1 ? 1 : P <= 2 ? 2 : P <= 4 ? 4 : P <= 8 ? 8 : P <= 16 ? 16 : P <= 32 ? 32 : 64; }
 template <int M, int NI> constexpr int fused_rows_per_block() {
     constexpr int tpr = NI * lane_slot<Plan<M>::P>();
     return tpr >= 256 ? 1 : (256 / tpr > 8 ? 8 : 256 / tpr);
@@ -97,14 +96,19 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
     // and the padded Nyquist real) -> M complex = 2M real values, each kept in
     // its input's region; every row load is issued before the first transform
     if (lane) {
+        // kc = N/2 + 1 with N = 4M/3: positions below KC carry data, the rest
+        // of the padded row is the zero band
+        constexpr int KC = 2 * M / 3 + 1;
         cx<T> raw[IPG][E];
 #pragma unroll
         for (int t = 0; t < IPG; ++t) {
-            const cx<T>* src = in + (g * IPG + t) * istride + (active ? row : 0) * (long)kc;
+            const cx<T>* src = in + (g * IPG + t) * istride + (active ? row : 0) * (long)KC;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = q + P * e;
-                raw[t][e] = (active && k < kc) ? src[k] : cx<T>{0, 0};
+                if (P * e + P <= KC) raw[t][e] = src[k];
+                else if (P * e < KC) raw[t][e] = k < KC ? src[k] : cx<T>{0, 0};
+                else raw[t][e] = cx<T>{0, 0};
             }
         }
 #pragma unroll
@@ -129,7 +133,7 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
                 v[e] = cx<T>{A.x - B.y, A.y + B.x};
             }
             WaveSync::sync();
-            fft_regs<T, M, false, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+            fft_regs_row<T, M, false, WaveSync>(v, q, lds, lm.base, tw);
 #pragma unroll
             for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = cscale(v[e], scale);
         }
@@ -145,7 +149,7 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
         if (p < M) {
             cx<T> x[NI];
 #pragma unroll
-            for (int f = 0; f < NI; ++f) x[f] = lds[(rb * NI + f) * RP + p + (p >> 4)];
+            for (int f = 0; f < NI; ++f) x[f] = lds[(rb * NI + f) * RP + p];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 double h[NI], o[NO];
@@ -163,7 +167,7 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
         const int p = tr + u * TPR;
         if (p < M) {
 #pragma unroll
-            for (int o = 0; o < NO; ++o) lds[(rb * NI + o) * RP + p + (p >> 4)] = cx<T>{acc[u][o][0], acc[u][o][1]};
+            for (int o = 0; o < NO; ++o) lds[(rb * NI + o) * RP + p] = cx<T>{acc[u][o][0], acc[u][o][1]};
         }
     }
     __syncthreads();
@@ -176,7 +180,7 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = lds[lm.at(q + P * e)];
             WaveSync::sync();
-            fft_regs<T, M, true, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+            fft_regs_row<T, M, true, WaveSync>(v, q, lds, lm.base, tw);
 #pragma unroll
             for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
             WaveSync::sync();
@@ -201,7 +205,7 @@ static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, i
                          const char* name) {
     constexpr int NI = PR::NI, NO = PR::NO, NG = NI / PR::IPG, SL = lane_slot<Plan<M>::P>();
     constexpr int RB = fused_rows_per_block<M, NG>(), RP = row_lds_pitch<M>();
-    MFIC_CHECK(kc <= M, MFIC_ERR_SIZE, "half-spectrum rows longer than the padded transform");
+    MFIC_CHECK(3 * (kc - 1) == 2 * M, MFIC_ERR_SIZE, "half-spectrum rows must hold N/2 + 1 values of a 3N/2 padded row");
     const size_t lds = (size_t)RB * NI * RP * sizeof(cx<T>);
     MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
@@ -256,7 +260,7 @@ k_rows_c2r_particles(const cx<T>* __restrict__ f0, const cx<T>* __restrict__ f1,
             v[e] = cx<T>{A.x - B.y, A.y + B.x};
         }
         WaveSync::sync();
-        fft_regs<T, M, false, LdsRowMap, WaveSync>(v, q, lds, lm, tw);
+        fft_regs_row<T, M, false, WaveSync>(v, q, lds, lm.base, tw);
 #pragma unroll
         for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = cscale(v[e], scale);
     }
@@ -266,7 +270,7 @@ k_rows_c2r_particles(const cx<T>* __restrict__ f0, const cx<T>* __restrict__ f1,
     double* out = dst + 3 * (pbase + (size_t)row * N);
     for (int idx = tr; idx < 3 * N; idx += TPR) {
         const int p = idx / 3, c = idx - 3 * p;
-        const cx<T> w = lds[(rb * 3 + c) * RP + (p >> 1) + ((p >> 1) >> 4)];
+        const cx<T> w = lds[(rb * 3 + c) * RP + (p >> 1)];
         double val = (double)((p & 1) ? w.y : w.x);
         if (is_pos) {
             const int ijk = c == 0 ? i : (c == 1 ? j : p);

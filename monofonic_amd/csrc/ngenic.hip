@@ -237,55 +237,85 @@ __device__ __forceinline__ bool float_rounding_settled(double y) {
     return (float)(y - d) == f && (float)(y + d) == f;
 }
 
+// one value of the white-noise field from a (phase, amplitude) draw pair
 template <typename T>
-__global__ void __launch_bounds__(64)
+__device__ __forceinline__ cx<T> ngenic_value(double phase, double ampl, const CrTables& tabs) {
+    double zr, zi;
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        const double a = sqrt(-log(ampl));
+        double sn, cs;
+        sincos(phase, &sn, &cs);
+        zr = a * cs;
+        zi = a * sn;
+        done = float_rounding_settled(zr) && float_rounding_settled(zi);
+    }
+    if (!done) {
+        const double a = sqrt(-cr_log(ampl, tabs));
+        double sn, cs;
+        cr_sincos(phase, &sn, &cs, tabs);
+        zr = a * cs;
+        zi = a * sn;
+    }
+    return cx<T>{(T)zr, (T)zi};
+}
+
+// One thread per (i, j) column draws its kz = 0 .. n/2 values in stream
+// order.  kz = 0 values (and their Hermitian mirrors) are stored directly;
+// kz >= 1 values go through an LDS tile of NGENIC_KT draws per column and
+// leave the workgroup as runs of NGENIC_KT consecutive elements per column,
+// so every cache line is written whole.
+constexpr int NGENIC_BS = 64, NGENIC_KT = 16;
+template <typename T>
+__global__ void __launch_bounds__(NGENIC_BS)
 k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, CrTables tabs) {
-    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= (long)n * n) return;
-    const int i = (int)(col / n), j = (int)(col % n);
+    __shared__ cx<T> tile[NGENIC_BS][NGENIC_KT + 1];
+    const long cols = (long)n * n;
+    const long col0 = (long)blockIdx.x * NGENIC_BS;
+    const long col = col0 + threadIdx.x;
+    const bool active = col < cols;
+    const int i = active ? (int)(col / n) : 0, j = active ? (int)(col % n) : 0;
     const int nc = n / 2 + 1, nh = n / 2;
     const int ii = i > 0 ? n - i : 0, jj = j > 0 ? n - j : 0;
+    const bool nyq_col = i == nh || j == nh;
     uint64_t x[12];
-    ranlux_seed_dev(x, table[(long)i * n + j]);
     int carry = 0, pos = 12;  // pos == 12: the next draw runs a cycle
-    for (int k = 0; k < nc; ++k) {
+    if (active) ranlux_seed_dev(x, table[(long)i * n + j]);
+    auto draw = [&](double& phase, double& ampl) {
         if (pos == 12) { ranlux_cycle202(x, carry); pos = 0; }
-        const double phase = (double)sel12(x, pos++) * (1.0 / 281474976710656.0) * 2 * 3.14159265358979323846;
-        double ampl;
+        phase = (double)sel12(x, pos++) * (1.0 / 281474976710656.0) * 2 * 3.14159265358979323846;
         do {
             if (pos == 12) { ranlux_cycle202(x, carry); pos = 0; }
             ampl = (double)sel12(x, pos++) * (1.0 / 281474976710656.0);
         } while (ampl == 0 || ampl == 1);
-        if (i == nh || j == nh || k == nh) continue;
-        if (i == 0 && j == 0 && k == 0) continue;
-        double zr, zi;
-        bool done = false;
-        if constexpr (sizeof(T) == 4) {
-            const double a = sqrt(-log(ampl));
-            double sn, cs;
-            sincos(phase, &sn, &cs);
-            zr = a * cs;
-            zi = a * sn;
-            done = float_rounding_settled(zr) && float_rounding_settled(zi);
+    };
+    if (active) {  // kz = 0: the draw fills (i, j) and its mirror (-i, -j)
+        double phase, ampl;
+        draw(phase, ampl);
+        if (!nyq_col && !(i == 0 && j == 0) && (i == 0 ? j < nh : i < nh)) {
+            const cx<T> z = ngenic_value<T>(phase, ampl, tabs);
+            g[((long)i * n + j) * nc] = z;
+            g[((long)ii * n + jj) * nc] = cx<T>{z.x, -z.y};
         }
-        if (!done) {
-            const double a = sqrt(-cr_log(ampl, tabs));
-            double sn, cs;
-            cr_sincos(phase, &sn, &cs, tabs);
-            zr = a * cs;
-            zi = a * sn;
-        }
-        if (k > 0) {
-            g[((long)i * n + j) * nc + k] = cx<T>{(T)zr, (T)zi};
-        } else if (i == 0) {
-            if (j < nh) {
-                g[((long)i * n + j) * nc] = cx<T>{(T)zr, (T)zi};
-                g[((long)i * n + jj) * nc] = cx<T>{(T)zr, (T)-zi};
+    }
+    for (int k0 = 1; k0 < nc; k0 += NGENIC_KT) {
+        if (active) {
+            for (int kk = 0; kk < NGENIC_KT; ++kk) {
+                const int k = k0 + kk;
+                if (k >= nc) break;
+                double phase, ampl;
+                draw(phase, ampl);
+                tile[threadIdx.x][kk] = (nyq_col || k == nh) ? cx<T>{0, 0} : ngenic_value<T>(phase, ampl, tabs);
             }
-        } else if (i < nh) {
-            g[((long)i * n + j) * nc] = cx<T>{(T)zr, (T)zi};
-            g[((long)ii * n + jj) * nc] = cx<T>{(T)zr, (T)-zi};
         }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < NGENIC_BS * NGENIC_KT; idx += NGENIC_BS) {
+            const int c = idx / NGENIC_KT, kk = idx % NGENIC_KT;
+            const long cc = col0 + c;
+            const int k = k0 + kk;
+            if (cc < cols && k < nc) g[cc * nc + k] = tile[c][kk];
+        }
+        __syncthreads();
     }
 }
 
@@ -325,7 +355,7 @@ void ngenic_fill(mfic_grid g, const uint32_t* dev_table) {
     static const CrTables tabs = make_tables();
     const int n = (int)g->n[0];
     const long cols = (long)n * n;
-    const int bs = 64;
+    const int bs = NGENIC_BS;
     ProfScope prof(g->ctx, "ngenic_fill", (double)cols * (n / 2 + 1) * (g->prec == MFIC_F64 ? 16 : 8));
     if (g->prec == MFIC_F64)
         hipLaunchKernelGGL(k_ngenic_fill<double>, dim3((unsigned)((cols + bs - 1) / bs)), dim3(bs), 0, g->ctx->stream,

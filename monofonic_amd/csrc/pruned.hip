@@ -351,7 +351,7 @@ k_rows_c2r_pruned(const cx<T>* __restrict__ kin, int kc, T* __restrict__ rout, l
         v[e] = cx<T>{A.x - B.y, A.y + B.x};
     }
     __syncthreads();
-    fft_regs<T, M, false>(v, q, lds, LdsRowMap{rw * RP}, tw);
+    fft_regs_row<T, M, false>(v, q, lds, rw * RP, tw);
     if (active) {
         cx<T>* dst = reinterpret_cast<cx<T>*>(rout + row * rpitch);
 #pragma unroll
@@ -405,7 +405,7 @@ k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long 
 #pragma unroll
         for (int u = 0; u < CH; ++u) v[e0 + u] = cx<T>{(T)s0[u], (T)s1[u]};
     }
-    fft_regs<T, M, true>(v, q, lds, LdsRowMap{rw * RP}, tw);
+    fft_regs_row<T, M, true>(v, q, lds, rw * RP, tw);
     const LdsRowMap lm{rw * RP};
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[lm.at(q + P * e)] = v[e];
