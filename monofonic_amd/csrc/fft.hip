@@ -37,7 +37,7 @@ k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? base[(long)(q + P * e) * ps] : cx<T>{0, 0};
-    fft_regs<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) base[(long)(q + P * e) * ps] = cscale(v[e], scale);
@@ -224,11 +224,11 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
     // segments per pencil position and one resident block per CU measured
     // best on MI355X for the 512 / 768 passes
     int tk = 64;
-    while (tk > 4 && ((size_t)L * tk * sizeof(cx<T>) > 139264 || P * tk > 1024)) tk /= 2;
+    while (tk > 4 && ((size_t)L * tk * pencil_lds_elem<T, true>() > pencil_lds_cap<true>() || P * tk > 1024)) tk /= 2;
     int pad = 0;
     const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) { tk = ov->second.first; pad = ov->second.second; }
-    while (tk > 1 && (size_t)L * (tk + pad) * sizeof(cx<T>) > max_lds(ctx)) tk /= 2;
+    while (tk > 1 && (size_t)L * (tk + pad) * pencil_lds_elem<T, true>() > max_lds(ctx)) tk /= 2;
     while (tk > 1 && P * tk > 1024) tk /= 2;
     const int tkp = tk + pad;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
@@ -236,7 +236,7 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
     const long ntiles = (long)gx * nouter;
     const bool xcd = nouter > 1;
     dim3 grid((unsigned)(xcd ? xcd_grid(ntiles) : ntiles));
-    size_t lds = (size_t)L * tkp * sizeof(cx<T>);
+    size_t lds = (size_t)L * tkp * pencil_lds_elem<T, true>();
     ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + (FWD ? "_fwd" : "_inv") + tag + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    2.0 * (double)tlen * L * nouter * sizeof(cx<T>));
     switch (tk) {

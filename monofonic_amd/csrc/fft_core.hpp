@@ -129,6 +129,9 @@ template <typename T, bool FWD> struct Dft<T, 12, FWD> {
 template <typename T, bool FWD> struct Dft<T, 16, FWD> {
     static __device__ __forceinline__ void run(cx<T>* a) { dft_composite<T, 4, 4, FWD>(a); }
 };
+template <typename T, bool FWD> struct Dft<T, 24, FWD> {
+    static __device__ __forceinline__ void run(cx<T>* a) { dft_composite<T, 8, 3, FWD>(a); }
+};
 
 // ------------------------------------------------------------ size plans --
 // radices r0 r1 r2 r3 (0 = unused), E = elements per thread
@@ -157,7 +160,7 @@ MFIC_PLAN(144, 12, 2, 12, 12, 0, 0)
 MFIC_PLAN(192, 12, 3, 4, 4, 12, 0)
 MFIC_PLAN(256, 16, 2, 16, 16, 0, 0)
 MFIC_PLAN(288, 12, 3, 2, 12, 12, 0)
-MFIC_PLAN(384, 12, 4, 4, 4, 12, 2)
+MFIC_PLAN(384, 24, 3, 24, 8, 2, 0)
 MFIC_PLAN(512, 16, 3, 16, 8, 4, 0)
 MFIC_PLAN(576, 24, 3, 8, 6, 12, 0)
 MFIC_PLAN(768, 24, 3, 12, 8, 8, 0)
@@ -233,9 +236,8 @@ MFIC_SWZ(192, 1, 1, 4, 4, 4, 0)
 MFIC_SWZ(256, 0, 1, 4, 1, 4, 0)
 MFIC_SWZ(288, 0, 1, 2, 0, 5, 1)
 MFIC_SWZ(288, 1, 1, 3, 0, 4, 7)
-MFIC_SWZ(384, 0, 1, 3, 1, 5, 1)
-MFIC_SWZ(384, 1, 1, 4, 4, 5, 2)
-MFIC_SWZ(384, 2, 1, 5, 0, 6, 6)
+MFIC_SWZ(384, 0, 1, 3, 0, 4, 1)
+MFIC_SWZ(384, 1, 1, 4, 0, 6, 3)
 MFIC_SWZ(512, 0, 1, 4, 1, 5, 1)
 MFIC_SWZ(512, 1, 1, 5, 0, 6, 8)
 #undef MFIC_SWZ
@@ -266,8 +268,11 @@ struct WaveSync {
     }
 };
 
-// one Stockham stage on registers v[E]; exchanges through LDS unless last
-template <typename T, int L, int S, bool FWD, class MAP, class SYNC = BlockSync>
+// one Stockham stage on registers v[E]; exchanges through LDS unless last.
+// SPLIT exchanges the real parts, then the imaginary parts, through an LDS
+// array of T: half the LDS of a complex exchange (more resident workgroups
+// for the wide pencil tiles) for twice the LDS instructions and barriers.
+template <typename T, int L, int S, bool FWD, class MAP, class SYNC = BlockSync, bool SPLIT = false>
 __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, const cx<T>* __restrict__ tw) {
     using PL = Plan<L>;
     constexpr int E = PL::E, P = PL::P, R = PL::r(S), B = E / R, NS = plan_ns<L>(S);
@@ -288,7 +293,7 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, co
             }
         }
         Dft<T, R, FWD>::run(a);
-        if (LAST) {  // outputs land on this thread's own positions
+        if (LAST || SPLIT) {  // outputs stay on this thread's registers
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b + r * B] = a[r];
         } else {
@@ -297,7 +302,31 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, co
             for (int r = 0; r < R; ++r) lds[m.at(base + r * NS)] = a[r];
         }
     }
-    if (!LAST) {
+    if constexpr (!LAST && SPLIT) {
+        T* f = reinterpret_cast<T*>(lds);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = q + P * b;
+            const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) f[m.at(base + r * NS)] = v[b + r * B].x;
+        }
+        SYNC::sync();
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e].x = f[m.at(q + P * e)];
+        SYNC::sync();
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = q + P * b;
+            const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) f[m.at(base + r * NS)] = v[b + r * B].y;
+        }
+        SYNC::sync();
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e].y = f[m.at(q + P * e)];
+        SYNC::sync();
+    } else if constexpr (!LAST) {
         SYNC::sync();
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = lds[m.at(q + P * e)];
@@ -306,13 +335,27 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, co
 }
 
 // full transform of length L on registers v[E] (positions q + P e)
-template <typename T, int L, bool FWD, class MAP, class SYNC = BlockSync>
+template <typename T, int L, bool FWD, class MAP, class SYNC = BlockSync, bool SPLIT = false>
 __device__ __forceinline__ void fft_regs(cx<T>* v, int q, cx<T>* lds, MAP m, const cx<T>* __restrict__ tw) {
     constexpr int S = Plan<L>::S;
-    fft_stage<T, L, 0, FWD, MAP, SYNC>(v, q, lds, m, tw);
-    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
-    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
-    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, MAP, SYNC>(v, q, lds, m, tw);
+    fft_stage<T, L, 0, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
+    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
+    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
+    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
+}
+
+// Pencil passes (TK neighbouring columns per workgroup) with one transform
+// exchange real and imaginary parts separately (SPLIT): with 16-column
+// tiles of 768-point pencils this lets three workgroups share a CU instead
+// of one.  Passes with two transforms per pencil (filters) keep complex
+// exchanges, whose barriers are half as many.
+template <typename T, bool SPLIT> constexpr size_t pencil_lds_elem() { return SPLIT ? sizeof(T) : sizeof(cx<T>); }
+// default LDS budget of a pencil tile: 128-byte row segments per pencil
+// position (16 complex FP32 columns) at 768 points, 32 at 512
+template <bool SPLIT> constexpr size_t pencil_lds_cap() { return SPLIT ? 69632 : 139264; }
+template <typename T, int L, bool FWD, bool SPLIT = true>
+__device__ __forceinline__ void fft_pencil(cx<T>* v, int q, cx<T>* lds, LdsMap m, const cx<T>* __restrict__ tw) {
+    fft_regs<T, L, FWD, LdsMap, BlockSync, SPLIT>(v, q, lds, m, tw);
 }
 
 // full transform of a row held by one thread group, exchanges through the

@@ -49,7 +49,7 @@ struct ProdSrc2Q {
 struct ProdSrc3 {
     // two inputs per lane group: 4-wave workgroups; registers for two waves
     // per SIMD (no spills), i.e. two workgroups per CU
-    static constexpr int NI = 16, NO = 4, IPG = 2, WAVES = 2;
+    static constexpr int NI = 16, NO = 4, IPG = 1, WAVES = 2;
     __device__ static __forceinline__ void eval(const double* h, double* o) {
         const double* a = h;
         const double* b = h + 6;

@@ -256,7 +256,7 @@ k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int
     for (int e = 0; e < E; ++e) v[e] = active ? ld.load(lc, q + P * e) : cx<T>{0, 0};
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = ld.fix(lc, q + P * e, v[e]);
-    fft_regs<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
         const auto sc = st.column(c);
 #pragma unroll
@@ -285,7 +285,7 @@ k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? ld.load(lc, q + P * e) : cx<T>{0, 0};
-    fft_regs<T, L, true>(v, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, true, false>(v, q, lds, LdsMap{t, tkp}, tw);
     const bool colzero = zero_kz_nyquist && (int)(c % kc) == n / 2;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -293,7 +293,7 @@ k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp
         const bool keep = !colzero && (m < n / 2 || m > n);
         v[e] = keep ? cscale(v[e], scale) : cx<T>{0, 0};
     }
-    fft_regs<T, L, false>(v, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, false, false>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
         const auto sc = st.column(c);
 #pragma unroll
@@ -431,13 +431,14 @@ k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long 
 
 // ------------------------------------------------------------- launchers --
 
-template <typename T, int L>
+template <typename T, int L, bool SPLIT>
 static int pick_tile(mfic_ctx ctx) {
     constexpr int P = Plan<L>::P;
     int tk = 64;
+    size_t cap = pencil_lds_cap<SPLIT>();
     const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
-    if (ov != ctx->tiles.end()) tk = ov->second.first;
-    while (tk > 1 && ((size_t)L * tk * sizeof(cx<T>) > 139264 || P * tk > 1024)) tk /= 2;
+    if (ov != ctx->tiles.end()) { tk = ov->second.first; cap = 139264; }
+    while (tk > 1 && ((size_t)L * tk * pencil_lds_elem<T, SPLIT>() > cap || P * tk > 1024)) tk /= 2;
     return tk;
 }
 
@@ -445,9 +446,9 @@ template <typename T, int L, bool FWD, class LD, class ST>
 static void launch_f(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
                      bool xcd = false) {
     constexpr int P = Plan<L>::P;
-    const int tk = pick_tile<T, L>(ctx);
+    const int tk = pick_tile<T, L, true>(ctx);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
-    const size_t lds = (size_t)L * tk * sizeof(cx<T>);
+    const size_t lds = (size_t)L * tk * pencil_lds_elem<T, true>();
     // axis-1 passes: 16-column tiles of rows with an odd pitch share cache
     // lines with their neighbours, which the XCD-aware order keeps in one L2
     const long ntiles = (ncols + tk - 1) / tk;
@@ -700,9 +701,9 @@ template <typename T, int L, class LD, class ST>
 static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, bool xcd, int n, int kc, int zk,
                           double scale, const char* name) {
     constexpr int P = Plan<L>::P;
-    const int tk = pick_tile<T, L>(ctx);
+    const int tk = pick_tile<T, L, false>(ctx);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
-    const size_t lds = (size_t)L * tk * sizeof(cx<T>);
+    const size_t lds = (size_t)L * tk * pencil_lds_elem<T, false>();
     const long ntiles = (ncols + tk - 1) / tk;
     dim3 grid((unsigned)(xcd ? xcd_grid(ntiles) : ntiles));
     ProfScope prof(ctx, std::string(name) + "_L" + std::to_string(L) + (sizeof(T) == 8 ? "_f64" : "_f32"),
