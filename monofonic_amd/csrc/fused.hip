@@ -141,8 +141,8 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
     __syncthreads();
 
     // 2. source products at each complex slot (two real points), FP64 sums
-    // rounded once to T
-    T acc[NPOS][NO][2];
+    // rounded once to T.  Slot p of every row is read and written by one
+    // thread only, so outputs overwrite the first NO input rows in place.
 #pragma unroll
     for (int u = 0; u < NPOS; ++u) {
         const int p = tr + u * TPR;
@@ -150,6 +150,7 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
             cx<T> x[NI];
 #pragma unroll
             for (int f = 0; f < NI; ++f) x[f] = lds[(rb * NI + f) * RP + p];
+            T res[NO][2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 double h[NI], o[NO];
@@ -157,17 +158,10 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
                 for (int f = 0; f < NI; ++f) h[f] = (double)(c ? x[f].y : x[f].x);
                 PR::eval(h, o);
 #pragma unroll
-                for (int r = 0; r < NO; ++r) acc[u][r][c] = (T)o[r];
+                for (int r = 0; r < NO; ++r) res[r][c] = (T)o[r];
             }
-        }
-    }
-    __syncthreads();
 #pragma unroll
-    for (int u = 0; u < NPOS; ++u) {
-        const int p = tr + u * TPR;
-        if (p < M) {
-#pragma unroll
-            for (int o = 0; o < NO; ++o) lds[(rb * NI + o) * RP + p] = cx<T>{acc[u][o][0], acc[u][o][1]};
+            for (int o = 0; o < NO; ++o) lds[(rb * NI + o) * RP + p] = cx<T>{res[o][0], res[o][1]};
         }
     }
     __syncthreads();
