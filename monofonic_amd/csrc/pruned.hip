@@ -28,14 +28,55 @@
 
 namespace mfic {
 
-// padded index -> source index (-1 in the zero band), convolution.hh:562-571
-__device__ __forceinline__ int src_of(int ip, int n) { return ip <= n / 2 ? ip : (ip > n ? ip - n / 2 : -1); }
-__device__ __forceinline__ int pad_of(int i, int n) { return i > n / 2 ? i + n / 2 : i; }
-__device__ __forceinline__ double gradc(int idx, int n, double kf) {
-    return idx == n / 2 ? 0.0 : ((double)idx - (double)(idx > n / 2) * n) * kf;
-}
-
 // ---------------------------------------------------------- loaders / stores
+//
+// Every padded pass runs pencils of length L = 3N/2 and every loader and
+// store below is instantiated with that L, so N, N/2 + 1 and the strides are
+// compile-time constants.  Element e of a pencil sits on lane q at position
+// pos = q + P e; once the element loop is unrolled, which side of the zero
+// band it lies on is a constant for all lanes except the two elements that
+// straddle N/2 and N, so band tests and gradient factors cost nothing for the
+// others.
+
+template <int L> struct PadGeo {
+    static constexpr int P = Plan<L>::P, N = 2 * L / 3, NH = N / 2, KC = NH + 1;
+    static_assert(3 * N == 2 * L, "padded length must be 3N/2");
+    // 0: every lane below N/2, 1: every lane above N, 2: every lane in the
+    // zero band N/2 < pos <= N, 3: mixed
+    static __device__ __forceinline__ constexpr int side(int e) {
+        return P * e + P - 1 < NH ? 0 : (P * e > N ? 1 : (P * e > NH && P * e + P - 1 <= N ? 2 : 3));
+    }
+    static __device__ __forceinline__ bool band(int q, int e) {
+        const int pos = q + P * e;
+        const int sd = side(e);
+        return sd == 2 || (sd == 3 && pos > NH && pos <= N);
+    }
+    // source index behind pos (convolution.hh:562-571), -1 in the band
+    static __device__ __forceinline__ int src(int q, int e) {
+        const int pos = q + P * e;
+        switch (side(e)) {
+        case 0: return pos;
+        case 1: return pos - NH;
+        case 2: return -1;
+        default: return pos <= NH ? pos : (pos > N ? pos - NH : -1);
+        }
+    }
+    // gradient factor of the source mode behind pos: signed index times kf,
+    // 0 at the source Nyquist N/2 (never used in the band)
+    template <typename T> static __device__ __forceinline__ T grad(int q, int e, T kf) {
+        const int pos = q + P * e;
+        switch (side(e)) {
+        case 0: return (T)q * kf + (T)(P * e) * kf;
+        case 1: return (T)q * kf + (T)(P * e - L) * kf;
+        default: return pos == NH ? T(0) : (T)(pos - (pos > NH ? L : 0)) * kf;
+        }
+    }
+};
+
+// gradient factor of an unpadded index
+template <typename T> __device__ __forceinline__ T grad_t(int idx, int n, T kf) {
+    return idx == n / 2 ? T(0) : (T)(idx - (idx > n / 2) * n) * kf;
+}
 
 // pass A: operand of an N-grid k-space field.  Along a pencil only the
 // axis-0 gradient g_i varies, so every operand multiplier is a polynomial
@@ -55,11 +96,12 @@ template <typename T> struct LdOperand {
         else if (b == 0) be += -sg * ga_;
         else ga += -sg * (ga_ * gb_);
     }
-    __device__ __forceinline__ Col column(long col) const {
+    template <int L> __device__ __forceinline__ Col column(long col) const {
+        using G = PadGeo<L>;
         Col c;
-        const int j = (int)(col / kc), k = (int)(col % kc);
+        const int j = (int)(col / G::KC), k = (int)(col % G::KC);
         c.p = f + col;
-        const double gj = gradc(j, n, kf), gk = gradc(k, n, kf);
+        const double gj = grad_t<double>(j, G::N, kf), gk = grad_t<double>(k, G::N, kf);
         c.al = c.be = c.ga = 0.0;
         c.imag = false;
         switch (kind) {
@@ -83,31 +125,20 @@ template <typename T> struct LdOperand {
         c.ga *= scale;
         return c;
     }
-    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
-        const int i = src_of(pos, n);
-        return i < 0 ? cx<T>{0, 0} : c.p[(long)i * n * kc];
+    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L>;
+        const int i = G::src(q, e);
+        return i < 0 ? cx<T>{0, 0} : c.p[(long)i * (G::N * G::KC)];
     }
-    __device__ __forceinline__ cx<T> fix(const Col& c, int pos, cx<T> v) const {
-        const int i = src_of(pos, n);
-        if (i < 0) return v;
-        const double g = gradc(i, n, kf);
+    template <int L> __device__ __forceinline__ cx<T> fix(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L>;
+        if (G::band(q, e)) return v;
+        const double g = G::template grad<double>(q, e, kf);
         const double s = (c.al * g + c.be) * g + c.ga;
         return c.imag ? cx<T>{(T)(-s * (double)v.y), (T)(s * (double)v.x)}
                       : cx<T>{(T)(s * (double)v.x), (T)(s * (double)v.y)};
     }
 };
-// gradient factor of the source index behind padded position pos, in the
-// working precision: source index pos (pos <= n/2) or pos - n/2 (pos > n),
-// signed pos or pos - 3n/2, 0 at the source Nyquist n/2 (the zero band
-// n/2 < pos <= n carries no data, its factor is never used)
-template <typename T> __device__ __forceinline__ T grad_pad(int pos, int n, T kf) {
-    const int s = pos - (pos > n / 2 ? 3 * n / 2 : 0);
-    return pos == n / 2 ? T(0) : (T)s * kf;
-}
-// the same for an unpadded index
-template <typename T> __device__ __forceinline__ T grad_t(int idx, int n, T kf) {
-    return idx == n / 2 ? T(0) : (T)(idx - (idx > n / 2) * n) * kf;
-}
 // pass A of the factorised Hessians: the N-grid field times g_i^p (p = 0..2,
 // the number of x derivatives) and the pad factor
 template <typename T> struct LdPow {
@@ -115,13 +146,14 @@ template <typename T> struct LdPow {
     int n, kc, p;
     T kf, scale;
     struct Col { const cx<T>* p; };
-    __device__ __forceinline__ Col column(long col) const { return Col{f + col}; }
-    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
-        const int i = src_of(pos, n);
-        return i < 0 ? cx<T>{0, 0} : c.p[(long)i * n * kc];
+    template <int L> __device__ __forceinline__ Col column(long col) const { return Col{f + col}; }
+    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L>;
+        const int i = G::src(q, e);
+        return i < 0 ? cx<T>{0, 0} : c.p[(long)i * (G::N * G::KC)];
     }
-    __device__ __forceinline__ cx<T> fix(const Col&, int pos, cx<T> v) const {
-        const T g = grad_pad<T>(pos, n, kf);
+    template <int L> __device__ __forceinline__ cx<T> fix(const Col&, int q, int e, cx<T> v) const {
+        const T g = PadGeo<L>::template grad<T>(q, e, kf);
         const T m = scale * (p > 0 ? g : T(1)) * (p > 1 ? g : T(1));
         return cx<T>{m * v.x, m * v.y};
     }
@@ -133,17 +165,19 @@ template <typename T> struct LdAx1Mul {
     int n, np, kc, q, r;
     T kf, sign;
     struct Col { const cx<T>* p; T m; };
-    __device__ __forceinline__ Col column(long col) const {
-        const long ip = col / kc;
-        const int k = (int)(col % kc);
-        const T gk = grad_t<T>(k, n, kf);
-        return Col{kb + ip * np * kc + k, r == 0 ? sign : (r == 1 ? sign * gk : sign * (gk * gk))};
+    template <int L> __device__ __forceinline__ Col column(long col) const {
+        using G = PadGeo<L>;
+        const long ip = col / G::KC;
+        const int k = (int)(col % G::KC);
+        const T gk = grad_t<T>(k, G::N, kf);
+        return Col{kb + ip * (L * G::KC) + k, r == 0 ? sign : (r == 1 ? sign * gk : sign * (gk * gk))};
     }
-    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
-        return src_of(pos, n) < 0 ? cx<T>{0, 0} : c.p[(long)pos * kc];
+    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int ln, int e) const {
+        using G = PadGeo<L>;
+        return G::band(ln, e) ? cx<T>{0, 0} : c.p[(long)(ln + G::P * e) * G::KC];
     }
-    __device__ __forceinline__ cx<T> fix(const Col& c, int pos, cx<T> v) const {
-        const T g = grad_pad<T>(pos, n, kf);
+    template <int L> __device__ __forceinline__ cx<T> fix(const Col& c, int ln, int e, cx<T> v) const {
+        const T g = PadGeo<L>::template grad<T>(ln, e, kf);
         const T m = c.m * (q > 0 ? g : T(1)) * (q > 1 ? g : T(1));
         return cx<T>{m * v.x, m * v.y};
     }
@@ -153,38 +187,44 @@ template <typename T> struct StPadCol {
     cx<T>* kb;
     int n, np, kc;
     struct Col { cx<T>* p; };
-    __device__ __forceinline__ Col column(long col) const {
-        const int j = (int)(col / kc), k = (int)(col % kc);
-        return Col{kb + (long)pad_of(j, n) * kc + k};
+    template <int L> __device__ __forceinline__ Col column(long col) const {
+        using G = PadGeo<L>;
+        const int j = (int)(col / G::KC), k = (int)(col % G::KC);
+        return Col{kb + (long)(j > G::NH ? j + G::NH : j) * G::KC + k};
     }
-    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const { c.p[(long)pos * np * kc] = v; }
+    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L>;
+        c.p[(long)(q + G::P * e) * (L * G::KC)] = v;
+    }
 };
 // passes B / B': axis-1 pencils of the compact buffer, column = (i', kz)
-template <typename T> struct LdAx1 {
+template <typename T, bool SKIP_BAND> struct LdAx1 {
     const cx<T>* kb;
-    int n, np, kc, skip_band;
     struct Col { const cx<T>* p; };
-    __device__ __forceinline__ Col column(long col) const {
-        const long ip = col / kc, k = col % kc;
-        return Col{kb + ip * np * kc + k};
+    template <int L> __device__ __forceinline__ Col column(long col) const {
+        using G = PadGeo<L>;
+        const long ip = col / G::KC, k = col % G::KC;
+        return Col{kb + ip * (L * G::KC) + k};
     }
-    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const {
-        if (skip_band && pos > n / 2 && pos <= n) return cx<T>{0, 0};
-        return c.p[(long)pos * kc];
+    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L>;
+        if (SKIP_BAND && G::band(q, e)) return cx<T>{0, 0};
+        return c.p[(long)(q + G::P * e) * G::KC];
     }
-    __device__ __forceinline__ cx<T> fix(const Col&, int, cx<T> v) const { return v; }
+    template <int L> __device__ __forceinline__ cx<T> fix(const Col&, int, int, cx<T> v) const { return v; }
 };
-template <typename T> struct StAx1 {
+template <typename T, bool DROP_BAND> struct StAx1 {
     cx<T>* kb;
-    int n, np, kc, drop_band;
     struct Col { cx<T>* p; };
-    __device__ __forceinline__ Col column(long col) const {
-        const long ip = col / kc, k = col % kc;
-        return Col{kb + ip * np * kc + k};
+    template <int L> __device__ __forceinline__ Col column(long col) const {
+        using G = PadGeo<L>;
+        const long ip = col / G::KC, k = col % G::KC;
+        return Col{kb + ip * (L * G::KC) + k};
     }
-    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const {
-        if (drop_band && pos > n / 2 && pos <= n) return;
-        c.p[(long)pos * kc] = v;
+    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L>;
+        if (DROP_BAND && G::band(q, e)) return;
+        c.p[(long)(q + G::P * e) * G::KC] = v;
     }
 };
 // pass A' load: padded column of the compact buffer for kept j
@@ -192,43 +232,48 @@ template <typename T> struct LdPadCol {
     const cx<T>* kb;
     int n, np, kc;
     struct Col { const cx<T>* p; };
-    __device__ __forceinline__ Col column(long col) const {
-        const int j = (int)(col / kc), k = (int)(col % kc);
-        return Col{kb + (long)pad_of(j, n) * kc + k};
+    template <int L> __device__ __forceinline__ Col column(long col) const {
+        using G = PadGeo<L>;
+        const int j = (int)(col / G::KC), k = (int)(col % G::KC);
+        return Col{kb + (long)(j > G::NH ? j + G::NH : j) * G::KC + k};
     }
-    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const { return c.p[(long)pos * np * kc]; }
-    __device__ __forceinline__ cx<T> fix(const Col&, int, cx<T> v) const { return v; }
+    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L>;
+        return c.p[(long)(q + G::P * e) * (L * G::KC)];
+    }
+    template <int L> __device__ __forceinline__ cx<T> fix(const Col&, int, int, cx<T> v) const { return v; }
 };
 // pass A' store: unpad rules + output operator (+ optional -1/k^2) into the N-grid
 template <typename T> struct StUnpad {
     cx<T>* out;
     int n, kc, op, invlap;
     double scale, coef, kf;
-    struct Col { cx<T>* p; int j, k; double kj, kk; bool nyq; };
-    __device__ __forceinline__ Col column(long col) const {
+    struct Col { cx<T>* p; double kj, kk; bool nyq; };
+    template <int L> __device__ __forceinline__ Col column(long col) const {
+        using G = PadGeo<L>;
         Col c;
-        c.j = (int)(col / kc);
-        c.k = (int)(col % kc);
+        const int j = (int)(col / G::KC), k = (int)(col % G::KC);
         c.p = out + col;
-        c.kj = ((double)c.j - (double)(c.j > n / 2) * n) * kf;
-        c.kk = ((double)c.k - (double)(c.k > n / 2) * n) * kf;
-        c.nyq = (c.j == n / 2) || (c.k == n / 2);
+        c.kj = ((double)j - (double)(j > G::NH) * G::N) * kf;
+        c.kk = ((double)k - (double)(k > G::NH) * G::N) * kf;
+        c.nyq = (j == G::NH) || (k == G::NH);
         return c;
     }
-    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const {
-        const int i = src_of(pos, n);
+    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L>;
+        const int i = G::src(q, e);
         if (i < 0) return;
         double re = 0.0, im = 0.0;
-        if (!(c.nyq || i == n / 2)) {
+        if (!(c.nyq || i == G::NH)) {
             re = (double)v.x * scale;
             im = (double)v.y * scale;
             if (invlap) {
-                const double ki = ((double)i - (double)(i > n / 2) * n) * kf;
+                const double ki = G::template grad<double>(q, e, kf);
                 const double k2 = ki * ki + c.kj * c.kj + c.kk * c.kk;
                 if (k2 > 0.0) { re = -re / k2; im = -im / k2; } else { re = 0.0; im = 0.0; }
             }
         }
-        cx<T>* dst = c.p + (long)i * n * kc;
+        cx<T>* dst = c.p + (long)i * (G::N * G::KC);
         if (op == MFIC_OP_ASSIGN) { *dst = cx<T>{(T)re, (T)im}; return; }
         const cx<T> o = *dst;
         if (op == MFIC_OP_ADD) *dst = cx<T>{(T)((double)o.x + re), (T)((double)o.y + im)};
@@ -236,7 +281,6 @@ template <typename T> struct StUnpad {
         else *dst = cx<T>{(T)((double)o.x + re * coef), (T)((double)o.y + im * coef)};
     }
 };
-
 // ------------------------------------------------------------- kernels ----
 
 template <typename T, int L, int TK, bool FWD, class LD, class ST>
@@ -249,18 +293,20 @@ k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int
     const long col = (xcd ? xcd_tile(blockIdx.x, gridDim.x) : (long)blockIdx.x) * TK + t;
     const bool active = col < ncols;
     const long c = active ? col : 0;
-    const auto lc = ld.column(c);
+    const auto lc = ld.template column<L>(c);
     cx<T> v[E];
-    // every load of the pencil is issued before any multiplier is applied
+    // every load of the pencil is issued before any multiplier is applied;
+    // lanes past the last column read column 0 and store nothing, so no load
+    // carries a branch
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = active ? ld.load(lc, q + P * e) : cx<T>{0, 0};
+    for (int e = 0; e < E; ++e) v[e] = ld.template load<L>(lc, q, e);
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = ld.fix(lc, q + P * e, v[e]);
+    for (int e = 0; e < E; ++e) v[e] = ld.template fix<L>(lc, q, e, v[e]);
     fft_pencil<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
-        const auto sc = st.column(c);
+        const auto sc = st.template column<L>(c);
 #pragma unroll
-        for (int e = 0; e < E; ++e) st.store(sc, q + P * e, v[e]);
+        for (int e = 0; e < E; ++e) st.template store<L>(sc, q, e, v[e]);
     }
 }
 
@@ -272,32 +318,33 @@ k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int
 // axis; the column mask zeroes kz = N/2 and `scale` carries the pad factors.
 template <typename T, int L, int TK, class LD, class ST>
 __global__ void __launch_bounds__(Plan<L>::P * TK)
-k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int xcd, int n, int kc,
-                 int zero_kz_nyquist, T scale) {
-    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int xcd, int zero_kz_nyquist,
+                 T scale) {
+    using G = PadGeo<L>;
+    constexpr int E = Plan<L>::E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int t = threadIdx.x % TK, q = threadIdx.x / TK;
     const long col = (xcd ? xcd_tile(blockIdx.x, gridDim.x) : (long)blockIdx.x) * TK + t;
     const bool active = col < ncols;
     const long c = active ? col : 0;
-    const auto lc = ld.column(c);
+    const auto lc = ld.template column<L>(c);
     cx<T> v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = active ? ld.load(lc, q + P * e) : cx<T>{0, 0};
+    for (int e = 0; e < E; ++e) v[e] = ld.template load<L>(lc, q, e);
     fft_pencil<T, L, true, false>(v, q, lds, LdsMap{t, tkp}, tw);
-    const bool colzero = zero_kz_nyquist && (int)(c % kc) == n / 2;
+    const bool colzero = zero_kz_nyquist && (int)(c % G::KC) == G::NH;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int m = q + P * e;
-        const bool keep = !colzero && (m < n / 2 || m > n);
+        // keep the source modes: not the zero band, not the source Nyquist
+        const bool keep = !colzero && !G::band(q, e) && q + G::P * e != G::NH;
         v[e] = keep ? cscale(v[e], scale) : cx<T>{0, 0};
     }
     fft_pencil<T, L, false, false>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
-        const auto sc = st.column(c);
+        const auto sc = st.template column<L>(c);
 #pragma unroll
-        for (int e = 0; e < E; ++e) st.store(sc, q + P * e, v[e]);
+        for (int e = 0; e < E; ++e) st.template store<L>(sc, q, e, v[e]);
     }
 }
 
@@ -306,15 +353,21 @@ template <typename T> struct LdAx0 {
     const cx<T>* kb;
     int np, kc;
     struct Col { const cx<T>* p; };
-    __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
-    __device__ __forceinline__ cx<T> load(const Col& c, int pos) const { return c.p[(long)pos * np * kc]; }
+    template <int L> __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
+    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L>;
+        return c.p[(long)(q + G::P * e) * (L * G::KC)];
+    }
 };
 template <typename T> struct StAx0 {
     cx<T>* kb;
     int np, kc;
     struct Col { cx<T>* p; };
-    __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
-    __device__ __forceinline__ void store(const Col& c, int pos, cx<T> v) const { c.p[(long)pos * np * kc] = v; }
+    template <int L> __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
+    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L>;
+        c.p[(long)(q + G::P * e) * (L * G::KC)] = v;
+    }
 };
 
 // compact half-spectrum rows (kc <= M+1 values, the rest zero) -> real rows
@@ -541,8 +594,8 @@ static void c2r_from_operand_t(mfic_ctx ctx, const OperandDesc& o, size_t n, dou
     const double kf = 2.0 * 3.14159265358979323846 / L;
     LdOperand<T> la{static_cast<const cx<T>*>(o.f), N, kc, o.kind, o.d[0], o.d[1], o.d[2], o.d[3], kf, rfac};
     StPadCol<T> sa{static_cast<cx<T>*>(kbuf), N, np, kc};
-    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 1};
-    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 0};
+    LdAx1<T, true> lb{static_cast<const cx<T>*>(kbuf)};
+    StAx1<T, false> sb{static_cast<cx<T>*>(kbuf)};
     const double cs = sizeof(cx<T>);
     const long colsA = (long)N * kc, colsB = (long)np * kc;
     switch (np) {
@@ -585,8 +638,8 @@ static void r2c_products_t(mfic_ctx ctx, const std::vector<ProdTerm>& terms, siz
     const double rfac = std::sqrt((double)np * np * np) / std::sqrt((double)N * N * N);
     const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
     const double kf = 2.0 * 3.14159265358979323846 / L;
-    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 0};
-    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 1};
+    LdAx1<T, false> lb{static_cast<const cx<T>*>(kbuf)};
+    StAx1<T, true> sb{static_cast<cx<T>*>(kbuf)};
     LdPadCol<T> la{static_cast<const cx<T>*>(kbuf), N, np, kc};
     StUnpad<T> sa{static_cast<cx<T>*>(out), N, kc, op, invlap ? 1 : 0, norm / rfac, coef, kf};
     const double cs = sizeof(cx<T>);
@@ -612,8 +665,8 @@ static void c2r_to_kbuf_t(mfic_ctx ctx, const OperandDesc& o, size_t n, double L
     const double kf = 2.0 * 3.14159265358979323846 / L;
     LdOperand<T> la{static_cast<const cx<T>*>(o.f), N, kc, o.kind, o.d[0], o.d[1], o.d[2], o.d[3], kf, rfac};
     StPadCol<T> sa{static_cast<cx<T>*>(kbuf), N, np, kc};
-    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 1};
-    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 0};
+    LdAx1<T, true> lb{static_cast<const cx<T>*>(kbuf)};
+    StAx1<T, false> sb{static_cast<cx<T>*>(kbuf)};
     const double cs = sizeof(cx<T>);
     const long colsA = (long)N * kc, colsB = (long)np * kc;
     switch (np) {
@@ -635,8 +688,8 @@ static void r2c_from_kbuf_t(mfic_ctx ctx, size_t n, double L, void* kbuf, void* 
     const double rfac = std::sqrt((double)np * np * np) / std::sqrt((double)N * N * N);
     const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
     const double kf = 2.0 * 3.14159265358979323846 / L;
-    LdAx1<T> lb{static_cast<const cx<T>*>(kbuf), N, np, kc, 0};
-    StAx1<T> sb{static_cast<cx<T>*>(kbuf), N, np, kc, 1};
+    LdAx1<T, false> lb{static_cast<const cx<T>*>(kbuf)};
+    StAx1<T, true> sb{static_cast<cx<T>*>(kbuf)};
     LdPadCol<T> la{static_cast<const cx<T>*>(kbuf), N, np, kc};
     StUnpad<T> sa{static_cast<cx<T>*>(out), N, kc, op, invlap ? 1 : 0, norm / rfac, coef, kf};
     const double cs = sizeof(cx<T>);
@@ -679,7 +732,7 @@ static void hessians_to_kbuf_t(mfic_ctx ctx, const void* f, size_t n, double L, 
             for (int h = 0; h < 6; ++h) {                                                                      \
                 if (PQR[h][0] != p) continue;                                                                  \
                 LdAx1Mul<T> lb{xb + p * stride, N, np, kc, PQR[h][1], PQR[h][2], (T)kf, (T)-1};                \
-                StAx1<T> sb{ob + h * ostride, N, np, kc, 0};                                                   \
+                StAx1<T, false> sb{ob + h * ostride};                                                   \
                 launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rBh", (double)colsB * (N + np) * cs, true);        \
             }                                                                                                  \
         }                                                                                                      \
@@ -698,8 +751,8 @@ void padded_hessians_to_kbuf(mfic_ctx ctx, int prec, const void* f, size_t n, do
 }
 
 template <typename T, int L, class LD, class ST>
-static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, bool xcd, int n, int kc, int zk,
-                          double scale, const char* name) {
+static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, bool xcd, int zk, double scale,
+                          const char* name) {
     constexpr int P = Plan<L>::P;
     const int tk = pick_tile<T, L, false>(ctx);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
@@ -713,7 +766,7 @@ static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, 
     case K:                                                                                                     \
         if constexpr (P * K <= 1024)                                                                            \
             hipLaunchKernelGGL((k_pencils_filter<T, L, K, LD, ST>), grid, dim3(P * K), lds, ctx->stream, ld, st, \
-                               ncols, tw, K, xcd ? 1 : 0, n, kc, zk, (T)scale);                                 \
+                               ncols, tw, K, xcd ? 1 : 0, zk, (T)scale);                                        \
         break;
         TKC(1) TKC(2) TKC(4) TKC(8) TKC(16) TKC(32) TKC(64)
 #undef TKC
@@ -728,16 +781,16 @@ static void truncate_repad_t(mfic_ctx ctx, size_t n, void* kbuf) {
     const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
     const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
     cx<T>* kb = static_cast<cx<T>*>(kbuf);
-    LdAx1<T> l1{kb, N, np, kc, 0};
-    StAx1<T> s1{kb, N, np, kc, 0};
+    LdAx1<T, false> l1{kb};
+    StAx1<T, false> s1{kb};
     LdAx0<T> l0{kb, np, kc};
     StAx0<T> s0{kb, np, kc};
     const long cols = (long)np * kc;
     switch (np) {
 #define X(S)                                                                                       \
     case S:                                                                                        \
-        launch_filter<T, S>(ctx, l1, s1, cols, true, N, kc, 1, norm, "fft_truncate_ax1");          \
-        launch_filter<T, S>(ctx, l0, s0, cols, false, N, kc, 0, 1.0, "fft_truncate_ax0");          \
+        launch_filter<T, S>(ctx, l1, s1, cols, true, 1, norm, "fft_truncate_ax1");       \
+        launch_filter<T, S>(ctx, l0, s0, cols, false, 0, 1.0, "fft_truncate_ax0");       \
         break;
         MFIC_PADDED_SIZES(X)
 #undef X
