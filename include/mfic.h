@@ -138,6 +138,17 @@ typedef struct {
     double w_0, w_a, Omega_k, alpha_s, beta_s;
     int zero_radiation;
     double zstart, ztarget;
+    /* Tabulated transfer function, the CAMB_file plugin
+     * (transfer_CAMB_file.cc:36-160, 195-199): when tk_n > 0, T(k) is the
+     * natural cubic spline of ln T over ln k through (tk_k[i], tk_T[i])
+     * (k in h/Mpc, increasing; T > 0, the total-matter column), with k
+     * clamped to >= tk_k[1] and the running factor taken at k h; the sigma_8
+     * integral spans [tk_k[1], tk_k[tk_n-2]].  tk_n = 0 selects Eisenstein &
+     * Hu.  Host arrays, read (and copied to the device) by each call that
+     * takes these parameters; a grid whose largest |k| exceeds the table
+     * returns MFIC_ERR_CONFIG where the reference's spline would fail. */
+    const double *tk_k, *tk_T;
+    size_t tk_n;
 } mfic_cosmo_params;
 /* cosmology::calculator scalars (cosmology_calculator.hh:167-223, 334-432) */
 typedef struct {

@@ -38,6 +38,7 @@ class CosmoParams(C.Structure):
         ("O_nu_norm", C.c_double), ("w_0", C.c_double), ("w_a", C.c_double), ("Omega_k", C.c_double),
         ("alpha_s", C.c_double), ("beta_s", C.c_double), ("zero_radiation", C.c_int),
         ("zstart", C.c_double), ("ztarget", C.c_double),
+        ("tk_k", C.POINTER(C.c_double)), ("tk_T", C.POINTER(C.c_double)), ("tk_n", C.c_size_t),
     ]
 
 

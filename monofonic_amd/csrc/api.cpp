@@ -385,8 +385,21 @@ int mfic_phi1_from_noise(mfic_grid phi, mfic_grid wn, const mfic_cosmo_params* p
         EhParams eh;
         mfic_cosmo_scalars s;
         cosmo_compute(*p, &s, &eh);
+        check_tk_range(*p, phi->n, phi->L);
         const double volfac = std::pow(phi->L[0] / (double)phi->n[0] / 2.0 / 3.14159265358979323846, 1.5);
-        phi1_from_noise(phi, wn, eh, volfac, fixing, inversion);
+        // a table is copied to the device for this call and released after
+        // the stream has consumed it
+        void* dtk = tk_upload(*p, &eh);
+        try {
+            phi1_from_noise(phi, wn, eh, volfac, fixing, inversion);
+        } catch (...) {
+            if (dtk) { (void)hipStreamSynchronize(phi->ctx->stream); (void)hipFree(dtk); }
+            throw;
+        }
+        if (dtk) {
+            MFIC_HIP(hipStreamSynchronize(phi->ctx->stream));
+            MFIC_HIP(hipFree(dtk));
+        }
         phi->space = MFIC_KSPACE;
     });
 }

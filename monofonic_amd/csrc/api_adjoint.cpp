@@ -42,8 +42,10 @@ struct mfic_lpt2_s {
     mfic_lpt_coef coef{};
     GridPtr w, phi1, phi2, lam, lmu, r, t[3];
     mfic_conv conv = nullptr;
+    void* d_tk = nullptr;  // spline nodes of a tabulated transfer function
     ~mfic_lpt2_s() {
         if (conv) (void)mfic_conv_destroy(conv);
+        if (d_tk) (void)hipFree(d_tk);
     }
 };
 
@@ -106,6 +108,12 @@ int mfic_lpt2_create(mfic_ctx ctx, size_t n, double L, int prec, const mfic_cosm
         p->unit = unit;
         mfic_cosmo_scalars sc;
         mfic::cosmo_compute(*cosmo, &sc, &p->eh);
+        {
+            const size_t n3[3] = {n, n, n};
+            const double L3[3] = {L, L, L};
+            mfic::check_tk_range(*cosmo, n3, L3);
+        }
+        p->d_tk = mfic::tk_upload(*cosmo, &p->eh);
         ck(mfic_lpt_coefficients(cosmo, n, L, 2, &p->coef));
         make(p.get(), p->w, MFIC_RSPACE);
         make(p.get(), p->phi1, MFIC_KSPACE);

@@ -39,6 +39,7 @@ struct mfic_run_s {
     mfic_grid comp[12] = {};  // particle mode: component fields per sub-lattice
     mfic_grid fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t* d_table = nullptr;
+    void* d_tk = nullptr;  // spline nodes of a tabulated transfer function
     double *d_pos = nullptr, *d_vel = nullptr;
     size_t npart = 0;
     mfic::PltTables plt;
@@ -109,6 +110,7 @@ mfic_run_s::~mfic_run_s() {
     for (auto& g : fields) if (g) mfic_grid_destroy(g);
     for (auto& g : comp) if (g) mfic_grid_destroy(g);
     if (d_table) (void)hipFree(d_table);
+    if (d_tk) (void)hipFree(d_tk);
     if (d_pos) (void)hipFree(d_pos);
     if (d_vel) (void)hipFree(d_vel);
 }
@@ -294,6 +296,15 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
         r->cfg = *cfg;
         r->cosmo = *cosmo;
         cosmo_compute(*cosmo, &r->sc, &r->eh);
+        {
+            const size_t n3[3] = {cfg->n, cfg->n, cfg->n};
+            const double L3[3] = {cfg->L, cfg->L, cfg->L};
+            check_tk_range(*cosmo, n3, L3);
+        }
+        r->d_tk = tk_upload(*cosmo, &r->eh);
+        // the caller's table arrays are borrowed for this call only
+        r->cosmo.tk_k = r->cosmo.tk_T = nullptr;
+        r->cosmo.tk_n = 0;
         compute_coefs(*cosmo, cfg->n, cfg->L, cfg->lpt_order, r->sc, &r->coef);
         const size_t n = cfg->n;
         for (int p = 0; p < (cfg->lpt_order == 1 ? 1 : cfg->lpt_order == 2 ? 2 : 6); ++p)

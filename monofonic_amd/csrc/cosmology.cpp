@@ -139,6 +139,7 @@ struct Calc {
     double Draw(double a) const { return std::exp(lnD(std::log(a))); }
     double f(double a) const { return std::exp(lnf(std::log(a))); }
 
+    std::vector<double> tk;  // host spline nodes of a tabulated transfer function
     double sig8_integrand(double k) const {
         const double x = k * 8.0;
         const double w = x < 0.001 ? 1.0 - 0.1 * x * x : 3.0 * (std::sin(x) - x * std::cos(x)) / (x * x * x);
@@ -165,7 +166,19 @@ struct Calc {
         s.Dplus_start = Draw(astart) / s.Dnow;
         s.Dplus_target = Draw(atarget) / s.Dnow;
         eh = eh_setup(p);
-        s.sigma8_measured = std::sqrt(4.0 * kPi * integrate(1e-4, 1e4, 1e-13));
+        // sigma_8 over the transfer function's own range: [1e-4, 1e4] h/Mpc
+        // for E&H (transfer_eisenstein.cc:245-253), the second to the
+        // next-to-last tabulated k for a table (transfer_CAMB_file.cc:156-158)
+        double kmin = 1e-4, kmax = 1e4;
+        if (p.tk_n > 0) {
+            tk = tk_spline_build(p.tk_k, p.tk_T, p.tk_n);
+            eh.tk = tk.data();
+            eh.tk_n = (int)p.tk_n;
+            eh.tk_lkmin = std::log(p.tk_k[1]);
+            kmin = p.tk_k[1];
+            kmax = p.tk_k[p.tk_n - 2];
+        }
+        s.sigma8_measured = std::sqrt(4.0 * kPi * integrate(kmin, kmax, 1e-13));
         if (p.A_s > -1.0)
             s.pnorm = p.A_s * std::pow(p.k_p / h, 1.0 - p.n_s) * (h * h * h * h) /
                       (4 * kPi * s.Dplus_target * s.Dplus_target);
@@ -270,6 +283,70 @@ void cosmo_defaults(mfic_cosmo_params* p) {
     p->sigma_8 = -1.0; p->zstart = 100.0; p->ztarget = 0.0;
 }
 
+std::vector<double> tk_spline_build(const double* k, const double* T, size_t n) {
+    if (n < 6 || !k || !T) throw std::invalid_argument("a tabulated transfer function needs at least 6 rows");
+    std::vector<double> x(n), y(n), h(n - 1), m(n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        if (!(k[i] > 0.0) || !(T[i] > 0.0))
+            throw std::invalid_argument("tabulated transfer function: k and T must be positive (row " +
+                                        std::to_string(i) + ")");
+        x[i] = std::log(k[i]);
+        y[i] = std::log(T[i]);
+        if (i > 0) {
+            h[i - 1] = x[i] - x[i - 1];
+            if (!(h[i - 1] > 0.0))
+                throw std::invalid_argument("tabulated transfer function: k must increase (row " + std::to_string(i) +
+                                            ")");
+        }
+    }
+    // natural spline: m_0 = m_{n-1} = 0 and, for 0 < i < n-1,
+    // h_{i-1} m_{i-1} + 2 (h_{i-1} + h_i) m_i + h_i m_{i+1} = 6 (s_i - s_{i-1}),
+    // s_i the slope of interval i; tridiagonal elimination then back substitution
+    std::vector<double> diag(n, 1.0), rhs(n, 0.0);
+    for (size_t i = 1; i + 1 < n; ++i) {
+        diag[i] = 2.0 * (h[i - 1] + h[i]);
+        rhs[i] = 6.0 * ((y[i + 1] - y[i]) / h[i] - (y[i] - y[i - 1]) / h[i - 1]);
+        if (i > 1) {
+            const double w = h[i - 1] / diag[i - 1];
+            diag[i] -= w * h[i - 1];
+            rhs[i] -= w * rhs[i - 1];
+        }
+    }
+    for (size_t i = n - 2; i >= 1; --i) m[i] = (rhs[i] - (i + 2 < n ? h[i] * m[i + 1] : 0.0)) / diag[i];
+    // per interval: ln T = a + b t + c t^2 + d t^3, t = ln k - ln k_i
+    std::vector<double> nodes(5 * n, 0.0);
+    for (size_t i = 0; i < n; ++i) {
+        double* c = nodes.data() + 5 * i;
+        c[0] = x[i];
+        c[1] = y[i];
+        if (i + 1 < n) {
+            c[2] = (y[i + 1] - y[i]) / h[i] - h[i] * (2.0 * m[i] + m[i + 1]) / 6.0;
+            c[3] = 0.5 * m[i];
+            c[4] = (m[i + 1] - m[i]) / (6.0 * h[i]);
+        }
+    }
+    return nodes;
+}
+
+void* tk_upload(const mfic_cosmo_params& p, EhParams* eh) {
+    if (p.tk_n == 0) {
+        eh->tk = nullptr;
+        eh->tk_n = 0;
+        return nullptr;
+    }
+    const std::vector<double> nodes = tk_spline_build(p.tk_k, p.tk_T, p.tk_n);
+    void* d = nullptr;
+    if (hipMalloc(&d, nodes.size() * sizeof(double)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+    if (hipMemcpy(d, nodes.data(), nodes.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        throw std::runtime_error("hipMemcpy failed");
+    }
+    eh->tk = static_cast<const double*>(d);
+    eh->tk_n = (int)p.tk_n;
+    eh->tk_lkmin = std::log(p.tk_k[1]);
+    return d;
+}
+
 void cosmo_compute(const mfic_cosmo_params& p, mfic_cosmo_scalars* s, EhParams* eh) {
     if (!(p.Omega_m > 0.0) || !(p.Omega_b > 0.0) || !(p.H0 > 0.0))
         throw std::invalid_argument("Omega_m, Omega_b and H0 must be positive");
@@ -279,7 +356,10 @@ void cosmo_compute(const mfic_cosmo_params& p, mfic_cosmo_scalars* s, EhParams* 
     c.p = p;
     c.run();
     if (s) *s = c.s;
-    if (eh) *eh = c.eh;
+    if (eh) {
+        *eh = c.eh;
+        eh->tk = nullptr;  // host nodes die with c: the device copy comes from tk_upload
+    }
 }
 
 void cosmo_eval(const mfic_cosmo_params& p, const std::string& what, const double* x, double* y, size_t n) {

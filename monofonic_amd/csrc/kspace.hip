@@ -196,6 +196,7 @@ void grid_phase_shift(mfic_grid g, const double s[3]) {
 }
 
 void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double volfac, int fixing, int inversion) {
+    MFIC_CHECK(eh.tk_n == 0 || eh.tk != nullptr, MFIC_ERR_STATE, "tabulated transfer function not on the device");
     const KGeom G = geom_of(phi);
     const size_t nc = (size_t)G.n0 * G.n1 * G.nc;
     const int bs = 256;

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <map>
@@ -122,6 +123,20 @@ template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int
 template <typename T>
 void fft_pencils(mfic_ctx ctx, void* data, int L, long ps, long os, int tlen, long nouter, bool fwd,
                  double scale, const char* tag = "");
+
+// a tabulated transfer function must reach the largest |k| of an n[0..2]
+// grid of sides L (h/Mpc); beyond its last row the reference's spline fails
+inline void check_tk_range(const mfic_cosmo_params& p, const size_t n[3], const double L[3]) {
+    if (p.tk_n == 0) return;
+    double k2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double ka = 3.14159265358979323846 * (double)n[a] / L[a];
+        k2 += ka * ka;
+    }
+    if (std::sqrt(k2) > p.tk_k[p.tk_n - 1])
+        throw Error(MFIC_ERR_CONFIG, "tabulated transfer function ends at k = " + std::to_string(p.tk_k[p.tk_n - 1]) +
+                                         " h/Mpc, the grid reaches " + std::to_string(std::sqrt(k2)));
+}
 
 // run f, translating exceptions into a status code and the thread-local
 // error message (no exception crosses the C ABI)

@@ -124,8 +124,18 @@ double orc_tk_eh(const orc_cosmo *c, double k) {
     return (c->eh[EH_fb] * tb + c->eh[EH_fc] * tc) * running_factor(c, k);
 }
 
+static double spline_eval(int n, const double *x, const double *y, const double *m, double xv);
+
+/* transfer_CAMB_file.cc:195-199, 244-245: k clamped to the second row, the
+ * running factor taken at k h */
+double orc_tk(const orc_cosmo *c, double k) {
+    if (c->tk_n <= 0) return orc_tk_eh(c, k);
+    double kc = k > c->tk_k[1] ? k : c->tk_k[1];
+    return exp(spline_eval(c->tk_n, c->tk_lx, c->tk_ly, c->tk_m, log(kc))) * running_factor(c, kc * c->h);
+}
+
 double orc_amplitude(const orc_cosmo *c, double k) {
-    return pow(k, 0.5 * c->n_s) * orc_tk_eh(c, k) * c->sqrtpnorm;
+    return pow(k, 0.5 * c->n_s) * orc_tk(c, k) * c->sqrtpnorm;
 }
 
 /* ------------------------------------------------------------ H(a) ----- */
@@ -291,7 +301,7 @@ static const double WG[4] = {
 static double dsigma8(const orc_cosmo *c, double k) {
     const double x = k * 8.0;
     const double w = (x < 0.001) ? 1.0 - 0.1 * x * x : 3.0 * (sin(x) - x * cos(x)) / (x * x * x);
-    double tf = orc_tk_eh(c, k);
+    double tf = orc_tk(c, k);
     return k * k * w * w * pow(k, c->n_s) * tf * tf;
 }
 
@@ -352,7 +362,23 @@ int orc_cosmo_init(orc_cosmo *c) {
     c->Dplus_target = D_raw(c, atarget) / c->Dnow;
 
     eh_set(c);
-    c->sigma8_measured = sqrt(4.0 * PI_ * orc_integrate_sigma8(c, 1e-4, 1e4, 1e-13));
+    double kmin = 1e-4, kmax = 1e4;   /* transfer_eisenstein.cc:245-253 */
+    if (c->tk_n > 0) {
+        if (c->tk_n < 6) return -1;
+        int n = c->tk_n;
+        c->tk_lx = (double *)malloc(sizeof(double) * n);
+        c->tk_ly = (double *)malloc(sizeof(double) * n);
+        c->tk_m = (double *)malloc(sizeof(double) * n);
+        for (int i = 0; i < n; ++i) {
+            if (!(c->tk_k[i] > 0.0) || !(c->tk_T[i] > 0.0)) return -1;
+            c->tk_lx[i] = log(c->tk_k[i]);
+            c->tk_ly[i] = log(c->tk_T[i]);
+        }
+        spline_init(n, c->tk_lx, c->tk_ly, c->tk_m);
+        kmin = c->tk_k[1];                /* transfer_CAMB_file.cc:156-158 */
+        kmax = c->tk_k[n - 2];
+    }
+    c->sigma8_measured = sqrt(4.0 * PI_ * orc_integrate_sigma8(c, kmin, kmax, 1e-13));
     if (!(c->A_s > -1.0))
         c->pnorm = c->sigma_8 * c->sigma_8 / (c->sigma8_measured * c->sigma8_measured);
     else
@@ -367,4 +393,6 @@ int orc_cosmo_init(orc_cosmo *c) {
 void orc_cosmo_free(orc_cosmo *c) {
     free(c->tab_loga); free(c->tab_logD); free(c->tab_logf); free(c->spl_D); free(c->spl_f);
     c->tab_loga = c->tab_logD = c->tab_logf = c->spl_D = c->spl_f = NULL;
+    free(c->tk_lx); free(c->tk_ly); free(c->tk_m);
+    c->tk_lx = c->tk_ly = c->tk_m = NULL;
 }

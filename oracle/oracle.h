@@ -89,6 +89,15 @@ typedef struct {
     /* growth table (cosmology_calculator.hh:92-152) */
     int ntab;
     double *tab_loga, *tab_logD, *tab_logf, *spl_D, *spl_f;
+    /* tabulated transfer function (transfer_CAMB_file.cc:36-160, 195-199):
+     * tk_n > 0 rows of (k [h/Mpc], T total matter), caller-owned; natural
+     * cubic spline of ln T in ln k, k clamped to >= k[1], running factor at
+     * k h, sigma_8 over [k[1], k[n-2]].  tk_n = 0: Eisenstein & Hu.
+     * pinned: scipy CubicSpline(bc_type="natural") on the same nodes; a
+     * dense table sampled from E&H reproduces the E&H sigma_8. */
+    int tk_n;
+    const double *tk_k, *tk_T;
+    double *tk_lx, *tk_ly, *tk_m;
 } orc_cosmo;
 
 /* Planck2018EE+BAO+SN defaults for every key (cosmology_parameters.cc:110-133,
@@ -102,6 +111,7 @@ double orc_cosmo_D(const orc_cosmo *c, double a);        /* D(a)/Dnow     */
 double orc_cosmo_f(const orc_cosmo *c, double a);
 double orc_cosmo_vfact(const orc_cosmo *c, double a);
 double orc_tk_eh(const orc_cosmo *c, double k_hmpc);      /* E&H total T(k) */
+double orc_tk(const orc_cosmo *c, double k_hmpc);         /* table or E&H   */
 double orc_amplitude(const orc_cosmo *c, double k_hmpc);  /* get_amplitude */
 /* adaptive Gauss-Kronrod integral of the sigma_8 integrand over [a,b] */
 double orc_integrate_sigma8(const orc_cosmo *c, double a, double b, double rel);

@@ -50,6 +50,8 @@ class Cosmo(C.Structure):
         ("ntab", C.c_int),
         ("tab_loga", C.c_void_p), ("tab_logD", C.c_void_p), ("tab_logf", C.c_void_p),
         ("spl_D", C.c_void_p), ("spl_f", C.c_void_p),
+        ("tk_n", C.c_int), ("tk_k", C.c_void_p), ("tk_T", C.c_void_p),
+        ("tk_lx", C.c_void_p), ("tk_ly", C.c_void_p), ("tk_m", C.c_void_p),
     ]
 
 
@@ -102,7 +104,7 @@ def lib():
         L.orc_cosmo_init.argtypes = [C.POINTER(Cosmo)]
         L.orc_cosmo_init.restype = C.c_int
         L.orc_cosmo_free.argtypes = [C.POINTER(Cosmo)]
-        for fn in ("orc_cosmo_H", "orc_cosmo_D", "orc_cosmo_f", "orc_cosmo_vfact", "orc_tk_eh",
+        for fn in ("orc_cosmo_H", "orc_cosmo_D", "orc_cosmo_f", "orc_cosmo_vfact", "orc_tk_eh", "orc_tk",
                    "orc_amplitude"):
             getattr(L, fn).restype = C.c_double
             getattr(L, fn).argtypes = [C.POINTER(Cosmo), C.c_double]
@@ -206,11 +208,20 @@ def ngenic_fill(seed: int, n: int) -> np.ndarray:
     return out
 
 
-def cosmology(**kw) -> Cosmo:
+def cosmology(transfer=None, **kw) -> Cosmo:
     """Cosmology with the reference defaults (ParameterSet=none) and overrides.
-    Giving sigma_8 selects sigma_8 normalisation (A_s = -1), as the reference."""
+    Giving sigma_8 selects sigma_8 normalisation (A_s = -1), as the reference.
+    transfer=(k, T): a tabulated transfer function (k in h/Mpc), kept alive
+    on the returned object."""
     c = Cosmo()
     lib().orc_cosmo_defaults(C.byref(c))
+    if transfer is not None:
+        k = np.ascontiguousarray(transfer[0], dtype=np.float64)
+        T = np.ascontiguousarray(transfer[1], dtype=np.float64)
+        c._tk_arrays = (k, T)
+        c.tk_n = len(k)
+        c.tk_k = k.ctypes.data
+        c.tk_T = T.ctypes.data
     for k, v in kw.items():
         if k in ("m_nu", "deg_nu"):
             for i, x in enumerate(v):
