@@ -150,33 +150,37 @@ struct AsmAllArgs {
     PltDev plt;
 };
 
-template <typename T>
+// ORDER, PLT and NSUB are compile-time so that every field load of the mode
+// is issued before the gradient (a CIC gather for PLT) is evaluated
+template <typename T, int ORDER, bool PLT, int NSUB>
 __global__ void k_assemble_all(AsmAllArgs a, size_t ntot) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= ntot) return;
+    cx<T> pv[ORDER], av[3];
+#pragma unroll
+    for (int o = 0; o < ORDER; ++o) pv[o] = static_cast<const cx<T>*>(a.phi[o])[idx];
+    if constexpr (ORDER > 2) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) av[d] = static_cast<const cx<T>*>(a.A3[d])[idx];
+    }
     const int k = (int)(idx % a.nc);
     const size_t r = idx / a.nc;
     const int j = (int)(r % a.n), i = (int)(r / a.n);
     double g[3], vc = 1.0;
-    if (a.plt.Dr) {
+    if constexpr (PLT) {
         plt_grad(a.plt, a.n, a.L, i, j, k, g, &vc);
     } else {
         g[0] = fgrad(i, a.n, a.kf);
         g[1] = fgrad(j, a.n, a.kf);
         g[2] = fgrad(k, a.n, a.kf);
     }
-    double pr[3] = {0, 0, 0}, pim[3] = {0, 0, 0};
-    for (int o = 0; o < a.order; ++o) {
-        const cx<T> v = static_cast<const cx<T>*>(a.phi[o])[idx];
-        pr[o] = (double)v.x;
-        pim[o] = (double)v.y;
-    }
     // compensated sums of c_o phi_o for displacement and velocity
     auto csum = [&](const double* cf, double& sr, double& si) {
         sr = si = 0.0;
         double er = 0.0, ei = 0.0;
-        for (int o = 0; o < a.order; ++o) {
-            const double xr = cf[o] * pr[o], xi = cf[o] * pim[o];
+#pragma unroll
+        for (int o = 0; o < ORDER; ++o) {
+            const double xr = cf[o] * (double)pv[o].x, xi = cf[o] * (double)pv[o].y;
             double t = sr + xr, bb = t - sr;
             er += (sr - (t - bb)) + (xr - bb);
             sr = t;
@@ -190,28 +194,22 @@ __global__ void k_assemble_all(AsmAllArgs a, size_t ntot) {
     double dr, di, vr, vi;
     csum(a.c, dr, di);
     csum(a.cv, vr, vi);
-    double Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0};
-    if (a.order > 2)
-        for (int d = 0; d < 3; ++d) {
-            const cx<T> v = static_cast<const cx<T>*>(a.A3[d])[idx];
-            Ar[d] = (double)v.x;
-            Ai[d] = (double)v.y;
-        }
     double ph_c = 1.0, ph_s = 0.0;
-    if (a.nsub > 1) {
+    if constexpr (NSUB > 1) {
         const double kx = ((double)i - (double)(i > a.n / 2) * a.n) * a.kf;
         const double ky = ((double)j - (double)(j > a.n / 2) * a.n) * a.kf;
         const double kz = ((double)k - (double)(k > a.n / 2) * a.n) * a.kf;
         sincos(a.sh * kx + a.sh * ky + a.sh * kz, &ph_s, &ph_c);
     }
+#pragma unroll
     for (int comp = 0; comp < 6; ++comp) {
         const bool vel = comp >= 3;
         const int d = comp % 3, dp = (d + 1) % 3, dpp = (d + 2) % 3;
         const double sr = vel ? vr : dr, si = vel ? vi : di;
         double xr = -g[d] * si, xi = g[d] * sr;
-        if (a.order > 2) {
-            const double cr = (-g[dp] * Ai[dpp]) - (-g[dpp] * Ai[dp]);
-            const double ci = (g[dp] * Ar[dpp]) - (g[dpp] * Ar[dp]);
+        if constexpr (ORDER > 2) {
+            const double cr = (-g[dp] * (double)av[dpp].y) - (-g[dpp] * (double)av[dp].y);
+            const double ci = (g[dp] * (double)av[dpp].x) - (g[dpp] * (double)av[dp].x);
             const double cA = vel ? a.cv[3] : a.c[3];
             xr += cA * cr;
             xi += cA * ci;
@@ -222,7 +220,7 @@ __global__ void k_assemble_all(AsmAllArgs a, size_t ntot) {
         xi *= u;
         if (idx == 0) { xr = 0.0; xi = 0.0; }
         if (a.out[comp]) static_cast<cx<T>*>(a.out[comp])[idx] = cx<T>{(T)xr, (T)xi};
-        if (a.nsub > 1 && a.out[comp + 6])
+        if (NSUB > 1 && a.out[comp + 6])
             static_cast<cx<T>*>(a.out[comp + 6])[idx] = cx<T>{(T)(xr * ph_c - xi * ph_s), (T)(xr * ph_s + xi * ph_c)};
     }
 }
@@ -314,10 +312,23 @@ void assemble_all(mfic_ctx ctx, int prec, size_t n, double L, int order, const d
     const size_t ntot = n * n * (n / 2 + 1);
     ProfScope prof(ctx, plt.Dr ? "assemble_all_plt" : "assemble_all",
                    (double)ntot * (prec == MFIC_F64 ? 16 : 8) * (order + (order > 2 ? 3 : 0) + nout));
-    if (prec == MFIC_F64)
-        hipLaunchKernelGGL(k_assemble_all<double>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, a, ntot);
-    else
-        hipLaunchKernelGGL(k_assemble_all<float>, dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, a, ntot);
+    MFIC_CHECK(order >= 1 && order <= 3 && (nsub == 1 || nsub == 2), MFIC_ERR_ARG, "assembly: bad order or sub-lattices");
+    const int key = (order - 1) * 4 + (plt.Dr ? 2 : 0) + (nsub - 1);
+    switch (key) {
+#define ASM(O, P, S)                                                                                             \
+    case (O - 1) * 4 + (P ? 2 : 0) + (S - 1):                                                                    \
+        if (prec == MFIC_F64)                                                                                    \
+            hipLaunchKernelGGL((k_assemble_all<double, O, P, S>), dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, \
+                               a, ntot);                                                                         \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_assemble_all<float, O, P, S>), dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, \
+                               a, ntot);                                                                         \
+        break;
+        ASM(1, false, 1) ASM(1, false, 2) ASM(1, true, 1) ASM(1, true, 2)
+        ASM(2, false, 1) ASM(2, false, 2) ASM(2, true, 1) ASM(2, true, 2)
+        ASM(3, false, 1) ASM(3, false, 2) ASM(3, true, 1) ASM(3, true, 2)
+#undef ASM
+    }
     MFIC_LAUNCH_CHECK();
 }
 
