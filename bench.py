@@ -10,7 +10,9 @@ Prints one JSON line.  Extra objects: "roofline" (dominant kernel, HIP-event
 timing on the library stream during the timed steps), "cpu_baseline" (the
 CPU FP64 oracle on a bounded 256^3 sample of the same configuration),
 "fp64" (same workload with FP64 bulk), "accuracy" (FP32 vs FP64 device
-fields, rel-L2 / max error per component) and "batch" (48 independent 2LPT
+fields, rel-L2 / max error per component), "accuracy_by_order" (the 256^3
+3LPT workload: FP32 vs FP64 error of Psi1, Psi2, Psi3 and the velocity
+separately), "batch" (48 independent 2LPT
 boxes of 128^3-256^3 replayed as one HIP graph with autotuned tiles) and
 "adjoint" (the hand-written adjoint of the 2LPT map under a CIC density loss
 at 256^3: timing, FP64 and finite-difference gradient checks).
@@ -310,6 +312,51 @@ def accuracy(ctx, n, box):
     return res
 
 
+def accuracy_by_order(ctx, n=256, box=500.0, z=24.0):
+    """The 256^3 3LPT workload (Orszag padding to 384^3, sc, no PLT) in FP32
+    and FP64 on the device, same seed: rel-L2 and max error of each order's
+    displacement (Psi1 = g1 grad phi1, Psi2 = g2 grad phi2, Psi3 = g3 grad
+    phi3 + g3c curl A3) and of the velocity sum (ic_generator.cc:939-1047),
+    against the FP64 run; the fields are formed on the host from the
+    downloaded potentials"""
+    import numpy as np
+    from monofonic_amd import F32, F64, lpt
+    from monofonic_amd.lpt import ICConfig, ICRun
+    coef = lpt.lpt_coefficients(lpt.baseline_cosmology(z), n, box, 3)
+    kf = 2 * np.pi / box
+    ki = np.fft.fftfreq(n, 1.0 / n)
+    ki[n // 2] = 0.0                       # gradient is 0 at the Nyquist index (grid_fft.hh:376-385)
+    kz = np.arange(n // 2 + 1, dtype=np.float64)
+    kz[n // 2] = 0.0
+    kvec = (ki[:, None, None] * kf, ki[None, :, None] * kf, kz[None, None, :] * kf)
+
+    def grad(f, d):
+        return np.fft.irfftn(1j * kvec[d] * f, s=(n, n, n), norm="ortho")
+
+    psi = {}
+    for prec in (F64, F32):
+        r = ICRun(ctx, ICConfig(n=n, box=box, zstart=z, lpt_order=3, prec=prec, seed=12345))
+        r.execute()
+        pot = [r.potential(w).download_k() for w in range(6)]
+        r.close()
+        out = {}
+        for d in range(3):
+            dp, dpp = (d + 1) % 3, (d + 2) % 3
+            out.setdefault("psi1", []).append(coef.g1 * grad(pot[0], d))
+            out.setdefault("psi2", []).append(coef.g2 * grad(pot[1], d))
+            out.setdefault("psi3", []).append(coef.g3 * grad(pot[2], d) +
+                                              coef.g3c * (grad(pot[3 + dpp], dp) - grad(pot[3 + dp], dpp)))
+        out["v"] = [coef.vfac1 * a + coef.vfac2 * b + coef.vfac3 * c
+                    for a, b, c in zip(out["psi1"], out["psi2"], out["psi3"])]
+        psi[prec] = {k: np.stack(v) for k, v in out.items()}
+    res = {}
+    for name in ("psi1", "psi2", "psi3", "v"):
+        a, b = psi[F32][name], psi[F64][name]
+        res[name] = {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                     "max_abs_rel": float(np.abs(a - b).max() / np.abs(b).max())}
+    return res
+
+
 def calibrate_copy(n=512, reps=10):
     """plain device-to-device copies of a known size (n^3 FP32 grid): under
     rocprofv3 --pmc they give the FETCH_SIZE / WRITE_SIZE scale factors"""
@@ -433,6 +480,9 @@ def main():
     if not args.no_accuracy:
         log("accuracy: FP32 vs FP64 device fields")
         line["accuracy"] = {"reference": "FP64 device run (same seed, same schedule)", **accuracy(ctx, n, box)}
+        log("accuracy by LPT order: 256^3 3LPT FP32 vs FP64")
+        line["accuracy_by_order"] = {"workload": "3LPT sc 256^3 (384^3 padded), box 500 Mpc/h, z_start 24, seed 12345",
+                                     "reference": "FP64 device run", **accuracy_by_order(ctx)}
     ctx.close()
     if not args.no_batch:
         log("batched boxes: 48 x 2LPT FP32 as one HIP graph")
