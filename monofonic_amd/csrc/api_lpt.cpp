@@ -362,15 +362,21 @@ int mfic_run_execute(mfic_run r, int blocking) {
         const int nsub = c.particles ? (1 << c.lattice) : 1;
         void* outs[12] = {};
         mfic_grid* og = c.particles ? r->comp : r->fields;
-        for (int q = 0; q < 6 * nsub; ++q) outs[q] = og[q]->d;
-        assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3, outs,
-                     nsub);
+        for (int q = 0; q < 6; ++q) outs[q] = og[q]->d;
+        // the second bcc sub-lattice is not assembled in k-space: its
+        // components come out of the first sub-lattice's axis-0 pass
+        assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3, outs, 1);
         if (c.particles) {
             // strided passes per component, then the contiguous pass fused
             // with the readout of the three components of each sub-lattice
-            for (int q = 0; q < 6 * nsub; ++q) {
-                if (c.prec == MFIC_F64) fft_c2r_cols<double>(r->ctx, (double*)og[q]->d, nn);
-                else fft_c2r_cols<float>(r->ctx, (float*)og[q]->d, nn);
+            for (int q = 0; q < 6; ++q) {
+                if (nsub > 1) {
+                    if (c.prec == MFIC_F64) fft_c2r_cols_shift<double>(r->ctx, (double*)og[q]->d, (double*)og[q + 6]->d, nn);
+                    else fft_c2r_cols_shift<float>(r->ctx, (float*)og[q]->d, (float*)og[q + 6]->d, nn);
+                } else {
+                    if (c.prec == MFIC_F64) fft_c2r_cols<double>(r->ctx, (double*)og[q]->d, nn);
+                    else fft_c2r_cols<float>(r->ctx, (float*)og[q]->d, nn);
+                }
             }
             for (int s = 0; s < nsub; ++s) {
                 const void* fp[3] = {og[6 * s]->d, og[6 * s + 1]->d, og[6 * s + 2]->d};

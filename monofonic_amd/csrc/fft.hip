@@ -44,6 +44,49 @@ k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
     }
 }
 
+// Axis-0 c2r pass of a k-space component that also produces the component
+// of the (1/2, 1/2, 1/2)-cell shifted sub-lattice: the shifted copy is the
+// column times exp(i pi (i' + j' + k') / n) (signed indices, shift_field of
+// grid_fft.hh:832-842 as particle_generator.hh:518-522 uses it), transformed
+// from the same loads.  Column t = j * nc + k of an [n0][n1][nc] grid.
+template <typename T, int L, int TK>
+__global__ void __launch_bounds__(Plan<L>::P * TK)
+k_pencils_ax0_shift(cx<T>* __restrict__ data, cx<T>* __restrict__ shifted, int n1, int nc, T scale,
+                    const cx<T>* __restrict__ tw, const cx<double>* __restrict__ half, int tkp) {
+    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int t = threadIdx.x % TK, q = threadIdx.x / TK;
+    const int tlen = n1 * nc;
+    const int tg = (int)blockIdx.x * TK + t;
+    const bool active = tg < tlen;
+    const int col = active ? tg : 0;
+    const long ps = (long)tlen;
+    const int j = col / nc, k = col - (col / nc) * nc;
+    const int mc = (j - (j > n1 / 2) * n1) + k;  // kz index k <= n/2 is its own signed index
+    cx<T> v[E], w[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = data[col + (long)(q + P * e) * ps];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = q + P * e;
+        int m = (i - (i > L / 2) * L) + mc;
+        m = m < 0 ? m + 2 * L : (m >= 2 * L ? m - 2 * L : m);
+        const cx<double> ph = half[m];
+        w[e] = cmul(v[e], cx<T>{(T)ph.x, (T)ph.y});
+    }
+    fft_pencil<T, L, false>(v, q, lds, LdsMap{t, tkp}, tw);
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) data[col + (long)(q + P * e) * ps] = cscale(v[e], scale);
+    }
+    fft_pencil<T, L, false>(w, q, lds, LdsMap{t, tkp}, tw);
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) shifted[col + (long)(q + P * e) * ps] = cscale(w[e], scale);
+    }
+}
+
 // Real rows of length 2M -> M+1 complex, in place (row pitch in reals).
 template <typename T, int M, int RW>
 __global__ void __launch_bounds__(Plan<M>::P * RW)
@@ -154,6 +197,14 @@ bool fft_length_supported(size_t n) {
 
 static std::vector<std::complex<long double>> make_table(int L, int kind) {
     std::vector<std::complex<long double>> tw;
+    if (kind == 2) {  // half-cell shift phases exp(+i pi m / L), m = 0 .. 2L-1
+        const long double PI = 3.141592653589793238462643383279502884L;
+        for (int m = 0; m < 2 * L; ++m) {
+            const long double a = PI * (long double)m / (long double)L;
+            tw.emplace_back(cosl(a), sinl(a));
+        }
+        return tw;
+    }
     switch (L) {
 #define X(S) case S: fill_tw<S>(tw); break;
         MFIC_FOR_SIZES(X)
@@ -350,6 +401,57 @@ template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], dou
 
 // the two strided c2r passes only (axis 0, axis 1): the contiguous-axis pass
 // is left to a fused row kernel
+template <typename T, int L>
+static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc) {
+    constexpr int P = Plan<L>::P;
+    int tk = 64;
+    while (tk > 4 && ((size_t)L * tk * pencil_lds_elem<T, true>() > pencil_lds_cap<true>() || P * tk > 1024)) tk /= 2;
+    const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    if (ov != ctx->tiles.end()) tk = ov->second.first;
+    while (tk > 1 && ((size_t)L * tk * pencil_lds_elem<T, true>() > max_lds(ctx) || P * tk > 1024)) tk /= 2;
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const cx<double>* half = static_cast<const cx<double>*>(twiddles(ctx, L, 2, MFIC_F64));
+    const int tlen = n1 * nc;
+    dim3 grid((unsigned)((tlen + tk - 1) / tk));
+    const size_t lds = (size_t)L * tk * pencil_lds_elem<T, true>();
+    ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + "_inv_ax0_shift" + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   3.0 * (double)tlen * L * sizeof(cx<T>));
+    switch (tk) {
+#define TKCASE(K)                                                                                            \
+    case K:                                                                                                  \
+        if constexpr (P * K <= 1024)                                                                         \
+            hipLaunchKernelGGL((k_pencils_ax0_shift<T, L, K>), grid, dim3(P * K), lds, ctx->stream, data, shifted, \
+                               n1, nc, (T)1, tw, half, K);                                                   \
+        break;
+        TKCASE(1) TKCASE(2) TKCASE(4) TKCASE(8) TKCASE(16) TKCASE(32) TKCASE(64)
+#undef TKCASE
+    default:
+        throw Error(MFIC_ERR_SIZE, "bad tile");
+    }
+    MFIC_LAUNCH_CHECK();
+}
+
+template <typename T> void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted, const size_t n[3]) {
+    check_sizes(n);
+    MFIC_CHECK(n[0] == n[1] && n[1] == n[2], MFIC_ERR_SIZE, "the shifted sub-lattice pass needs a cubic grid");
+    const long nc = (long)(n[2] / 2 + 1);
+    switch ((int)n[0]) {
+#define X(S)                                                                                                      \
+    case S:                                                                                                       \
+        launch_ax0_shift<T, S>(ctx, reinterpret_cast<cx<T>*>(data), reinterpret_cast<cx<T>*>(shifted), (int)n[1], \
+                               (int)nc);                                                                          \
+        break;
+        MFIC_FOR_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported FFT length " + std::to_string(n[0]));
+    }
+    for (T* d : {data, shifted})
+        fft_pencils<T>(ctx, d, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], false, 1.0, "_ax1");
+}
+template void fft_c2r_cols_shift<float>(mfic_ctx, float*, float*, const size_t*);
+template void fft_c2r_cols_shift<double>(mfic_ctx, double*, double*, const size_t*);
+
 template <typename T> void fft_c2r_cols(mfic_ctx ctx, T* data, const size_t n[3]) {
     check_sizes(n);
     const long nc = (long)(n[2] / 2 + 1);

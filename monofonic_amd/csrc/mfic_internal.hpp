@@ -106,7 +106,8 @@ class ProfScope {
 void* ctx_scratch(mfic_ctx ctx, const std::string& key, size_t bytes);
 
 // twiddles: complex transform of length L (kind 0), or real rows of length
-// 2M (kind 1: complex length M table followed by M+1 untangle factors)
+// 2M (kind 1: complex length M table followed by M+1 untangle factors), or
+// the half-cell shift phases exp(i pi m / L), m < 2L (kind 2, FP64)
 const void* twiddles(mfic_ctx ctx, int L, int kind, int prec);
 
 // FFT entry points on raw device buffers of layout [n0][n1][n2+2]
@@ -116,6 +117,9 @@ bool fft_length_supported(size_t n);
 
 // pass-level entry points used by fused pipelines
 template <typename T> void fft_c2r_cols(mfic_ctx ctx, T* data, const size_t n[3]);
+// fft_c2r_cols of a cubic-grid component plus, from the same axis-0 loads,
+// the component of the (1/2,1/2,1/2)-cell shifted sub-lattice into `shifted`
+template <typename T> void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted, const size_t n[3]);
 template <typename T> void fft_rows_r2c(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
 template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
 // complex pencils of length L: element (o, p, t) at data + o*os + p*ps + t,
