@@ -58,7 +58,11 @@ typedef struct mfic_grid_s *mfic_grid;
 const char *mfic_last_error(void);
 const char *mfic_version(void);
 /* context = one HIP device + one stream + twiddle/plan caches.
- * Replaces the process-global FFTW state set up in main.cc:156-182. */
+ * Replaces the process-global FFTW state set up in main.cc:156-182.
+ * Destruction order: every handle created on a context (grids, wrapped
+ * grids, convolvers, PLT tables, runs, graphs, 2LPT plans) must be destroyed
+ * before the context; mfic_ctx_destroy returns MFIC_ERR_STATE and leaves the
+ * context intact while any of them is alive. */
 int mfic_ctx_create(int device, mfic_ctx *out);
 int mfic_ctx_destroy(mfic_ctx ctx);
 int mfic_ctx_sync(mfic_ctx ctx);

@@ -138,15 +138,18 @@ def lpt2_map(noise, plan: LPT2):
 def cic_density(psi, box: float):
     """cloud-in-cell density (particles per cell, mean 1) on the n^3 mesh of
     one particle per cell displaced by psi (3, n, n, n) in Mpc/h;
-    differentiable in psi through torch autograd"""
+    differentiable in psi through torch autograd.  The cell index and the
+    CIC weights come from the displacement alone (in cells), and the
+    particle's integer lattice index is added afterwards, so the weights keep
+    the displacement's own precision instead of that of q + psi."""
     torch = _torch()
     n = psi.shape[1]
-    ar = torch.arange(n, device=psi.device, dtype=psi.dtype)
+    ar = torch.arange(n, device=psi.device, dtype=torch.long)
     q = torch.stack(torch.meshgrid(ar, ar, ar, indexing="ij"))
-    x = q + psi * (n / box)
-    i0 = torch.floor(x)
-    f = x - i0
-    i0 = i0.long()
+    s = psi * (n / box)
+    i0 = torch.floor(s)
+    f = s - i0
+    i0 = i0.long() + q
     rho = torch.zeros(n * n * n, dtype=psi.dtype, device=psi.device)
     for dx in (0, 1):
         wx = f[0] if dx else 1 - f[0]

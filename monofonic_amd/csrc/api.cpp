@@ -108,6 +108,10 @@ int mfic_ctx_create(int device, mfic_ctx* out) {
 int mfic_ctx_destroy(mfic_ctx ctx) {
     return guard([&] {
         if (!ctx) return;
+        const long live = ctx->children.load();
+        MFIC_CHECK(live == 0, MFIC_ERR_STATE,
+                   "context still has " + std::to_string(live) +
+                       " live handle(s) (grids, convolvers, PLT tables, runs, graphs, 2LPT plans): destroy them first");
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
         for (auto& kv : ctx->tables) (void)hipFree(kv.second.dev);
@@ -189,6 +193,7 @@ int mfic_grid_create(mfic_ctx ctx, const size_t n[3], const double L[3], int pre
             delete g;
             throw Error(MFIC_ERR_ALLOC, "device allocation of " + std::to_string(n[0] * n[1] * (n[2] + 2) * (prec ? 8 : 4)) + " bytes failed");
         }
+        ++ctx->children;
         grid_fill(g, 0.0);
         *out = g;
     });
@@ -199,6 +204,7 @@ int mfic_grid_destroy(mfic_grid g) {
         if (!g) return;
         (void)hipStreamSynchronize(g->ctx->stream);
         if (g->owns && g->d) (void)hipFree(g->d);
+        --g->ctx->children;
         delete g;
     });
 }

@@ -90,6 +90,7 @@ int mfic_grid_wrap(mfic_ctx ctx, const size_t n[3], const double L[3], int prec,
         g->nreal = n[0] * n[1] * (n[2] + 2);
         g->d = device_ptr;
         g->owns = false;
+        ++ctx->children;
         *out = g;
     });
 }
@@ -125,6 +126,7 @@ int mfic_lpt2_create(mfic_ctx ctx, size_t n, double L, int prec, const mfic_cosm
         const size_t n3[3] = {n, n, n};
         const double L3[3] = {L, L, L};
         ck(mfic_conv_create(ctx, n3, L3, prec, &p->conv));
+        ++ctx->children;
         *out = p.release();
     });
 }
@@ -133,7 +135,9 @@ int mfic_lpt2_destroy(mfic_lpt2 p) {
     return mfic::guard_call([&] {
         if (!p) return;
         (void)hipStreamSynchronize(p->ctx->stream);
+        mfic_ctx ctx = p->ctx;
         delete p;
+        --ctx->children;
     });
 }
 

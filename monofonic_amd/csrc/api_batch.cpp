@@ -46,6 +46,7 @@ int mfic_graph_end(mfic_ctx ctx, mfic_graph* out) {
             delete r;
             throw mfic::Error(MFIC_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
         }
+        ++ctx->children;
         *out = r;
     });
 }
@@ -65,7 +66,9 @@ int mfic_graph_destroy(mfic_graph g) {
         (void)hipStreamSynchronize(g->ctx->stream);
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
+        mfic_ctx ctx = g->ctx;
         delete g;
+        --ctx->children;
     });
 }
 
@@ -97,7 +100,12 @@ int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double* best_
         cfg.seed = 1;
         cfg.lunit = cfg.vunit = 1.0;
         mfic_run run = nullptr;
-        if (mfic_run_create(ctx, &cfg, &cp, &run) != MFIC_OK) throw mfic::Error(MFIC_ERR_HIP, mfic_last_error());
+        // a failing call's own status code is passed on (a size or
+        // configuration error is not a HIP error)
+        auto ck = [](int rc) {
+            if (rc != MFIC_OK) throw mfic::Error(rc, mfic_last_error());
+        };
+        ck(mfic_run_create(ctx, &cfg, &cp, &run));
         hipEvent_t a, b;
         MFIC_HIP(hipEventCreate(&a));
         MFIC_HIP(hipEventCreate(&b));
@@ -105,7 +113,7 @@ int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double* best_
             float best = 1e30f;
             for (int rep = 0; rep < 3; ++rep) {
                 MFIC_HIP(hipEventRecord(a, ctx->stream));
-                if (mfic_run_execute(run, 0) != MFIC_OK) throw mfic::Error(MFIC_ERR_HIP, mfic_last_error());
+                ck(mfic_run_execute(run, 0));
                 MFIC_HIP(hipEventRecord(b, ctx->stream));
                 MFIC_HIP(hipEventSynchronize(b));
                 float ms = 0.f;
@@ -116,7 +124,7 @@ int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double* best_
         };
         double result = 0.0;
         try {
-            if (mfic_run_execute(run, 1) != MFIC_OK) throw mfic::Error(MFIC_ERR_HIP, mfic_last_error());
+            ck(mfic_run_execute(run, 1));
             struct Knob { int length, kind; std::vector<int> cand; };
             std::vector<Knob> knobs = {{(int)n, 0, {4, 8, 16, 32, 64}}, {(int)n / 2, 1, {1, 2, 4, 8, 16}}};
             if (lpt_order > 1) knobs.push_back({(int)(3 * n / 2), 0, {4, 8, 16, 32, 64}});
@@ -144,7 +152,7 @@ int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double* best_
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        MFIC_CHECK(mfic_run_destroy(run) == MFIC_OK, MFIC_ERR_HIP, "run teardown failed");
+        ck(mfic_run_destroy(run));
         if (best_ms) *best_ms = result;
     });
 }

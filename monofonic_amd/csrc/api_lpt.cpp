@@ -25,6 +25,7 @@ struct mfic_conv_s {
 };
 
 struct mfic_plt_s {
+    mfic_ctx ctx = nullptr;
     mfic::PltTables t;
 };
 
@@ -149,6 +150,7 @@ int mfic_conv_create(mfic_ctx ctx, const size_t n[3], const double L[3], int pre
         c->r1 = dmalloc(c->core->real_reals() * es, "convolver padded real buffer");
         c->r2 = dmalloc(c->core->real_reals() * es, "convolver padded real buffer");
         c->fbuf = dmalloc(c->core->grid_reals() * es, "convolver N-grid buffer");
+        ++ctx->children;
         *out = c.release();
     });
 }
@@ -156,8 +158,10 @@ int mfic_conv_create(mfic_ctx ctx, const size_t n[3], const double L[3], int pre
 int mfic_conv_destroy(mfic_conv c) {
     return guard_call([&] {
         if (!c) return;
-        (void)hipStreamSynchronize(c->core->ctx->stream);
+        mfic_ctx ctx = c->core->ctx;
+        (void)hipStreamSynchronize(ctx->stream);
         delete c;
+        --ctx->children;
     });
 }
 
@@ -217,13 +221,20 @@ int mfic_plt_create(mfic_ctx ctx, int lattice, int ng, mfic_plt* out) {
     return guard_call([&] {
         MFIC_CHECK(ctx && out, MFIC_ERR_ARG, "null argument");
         auto p = std::make_unique<mfic_plt_s>();
+        p->ctx = ctx;
         plt_build(ctx, lattice, ng, p->t);
+        ++ctx->children;
         *out = p.release();
     });
 }
 
 int mfic_plt_destroy(mfic_plt p) {
-    return guard_call([&] { delete p; });
+    return guard_call([&] {
+        if (!p) return;
+        mfic_ctx ctx = p->ctx;
+        delete p;
+        --ctx->children;
+    });
 }
 
 int mfic_plt_download(mfic_plt p, double* Dr, double* Dtheta, double* Dphi, double* vfac) {
@@ -327,6 +338,7 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
             r->has_plt = true;
         }
         MFIC_HIP(hipStreamSynchronize(ctx->stream));
+        ++ctx->children;
         *out = r.release();
     });
 }
@@ -334,8 +346,10 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
 int mfic_run_destroy(mfic_run r) {
     return guard_call([&] {
         if (!r) return;
-        (void)hipStreamSynchronize(r->ctx->stream);
+        mfic_ctx ctx = r->ctx;
+        (void)hipStreamSynchronize(ctx->stream);
         delete r;
+        --ctx->children;
     });
 }
 

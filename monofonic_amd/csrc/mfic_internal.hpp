@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -70,6 +71,10 @@ struct mfic_ctx_s {
     bool prof = false;
     std::vector<mfic_prof_rec> prof_recs;
     std::vector<hipEvent_t> prof_pool;
+    // live handles created on this context (grids, convolvers, PLT tables,
+    // runs, graphs, 2LPT plans): mfic_ctx_destroy refuses while any remain,
+    // since each of them uses the context's stream when it is destroyed
+    std::atomic<long> children{0};
 };
 
 struct mfic_grid_s {

@@ -428,3 +428,11 @@ int orc_ic_run(const orc_ic_cfg *cfg, orc_cosmo *c, double *fields6, double *pos
     if (plt) orc_plt_free(plt);
     return 0;
 }
+
+/* ---------------------------------------------- exported k-space helpers */
+/* apply_InverseLaplacian (grid_fft.hh:440-449): -f/|k|^2, then DC := 0 */
+void orc_inverse_laplacian(int n, double L, double *f) { inverse_laplacian(n, L, f); }
+/* zero_DC_mode, k-space branch (grid_fft.hh:844-850) */
+void orc_zero_dc(double *f) { f[0] = 0.0; f[1] = 0.0; }
+/* operator*= (grid_fft.hh:400-412): every stored real times s */
+void orc_scale(int n, double *f, double s) { scale(n, f, s); }

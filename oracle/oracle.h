@@ -179,6 +179,13 @@ void orc_lattice_set_velocities(int n, double L, int lattice, int idim,
                                 double *field, double *vel);
 /* shift_field (grid_fft.hh:832-842): r-space in, r-space out */
 void orc_shift_field(int n, double L, const double s[3], double *field);
+/* apply_InverseLaplacian (grid_fft.hh:440-449), zero_DC_mode (grid_fft.hh:
+ * 844-850) and operator*= (grid_fft.hh:400-412) on an n^3 k-space buffer.
+ * pinned: inverse Laplacian through the potentials' brute-force checks
+ * (tests/test_oracle_orszag.py); the other two are single assignments. */
+void orc_inverse_laplacian(int n, double L, double *f);
+void orc_zero_dc(double *f);
+void orc_scale(int n, double *f, double s);
 
 /* Whole run, generic (field_lagrangian) output: 6 fields dx,dy,dz,vx,vy,vz,
  * each an n x n x (n+2) real buffer, or particle output when pos != NULL. */

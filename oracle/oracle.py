@@ -121,6 +121,9 @@ def lib():
         L.orc_lpt_component_k.argtypes = [C.POINTER(IcCfg), C.POINTER(LptCoef), C.c_void_p] + \
             [C.c_void_p] * 6 + [C.c_int, _dp]
         L.orc_shift_field.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double * 3), _dp]
+        L.orc_inverse_laplacian.argtypes = [C.c_int, C.c_double, _dp]
+        L.orc_zero_dc.argtypes = [_dp]
+        L.orc_scale.argtypes = [C.c_int, _dp, C.c_double]
         L.orc_lattice_set_positions.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp]
         L.orc_lattice_set_velocities.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp]
         L.orc_ic_run.argtypes = [C.POINTER(IcCfg), C.POINTER(Cosmo), C.c_void_p, C.c_void_p,
@@ -304,6 +307,35 @@ def ic_run(cfg: IcCfg, cosmo: Cosmo, particles: bool = False):
     if rc != 0:
         raise RuntimeError("oracle ic_run failed")
     return fields, pos, vel, st
+
+
+def shift_field(a: np.ndarray, L: float, s) -> None:
+    """shift_field in place on an (n, n, n+2) real-space array"""
+    lib().orc_shift_field(a.shape[0], L, C.byref((C.c_double * 3)(*s)), a)
+
+
+def inverse_laplacian(a: np.ndarray, L: float) -> None:
+    lib().orc_inverse_laplacian(a.shape[0], L, a)
+
+
+def zero_dc(a: np.ndarray) -> None:
+    lib().orc_zero_dc(a)
+
+
+def scale(a: np.ndarray, s: float) -> None:
+    lib().orc_scale(a.shape[0], a, s)
+
+
+def lattice_readout(field: np.ndarray, L: float, lattice: int, idim: int, is_pos: bool, lunit: float = 1.0):
+    """set_positions / set_velocities of one component (field is shifted in
+    place for bcc); returns the (npart, 3) array with column idim filled"""
+    n = field.shape[0]
+    out = np.zeros(3 * (1 << lattice) * n ** 3)
+    if is_pos:
+        lib().orc_lattice_set_positions(n, L, lattice, idim, lunit, field, out)
+    else:
+        lib().orc_lattice_set_velocities(n, L, lattice, idim, field, out)
+    return out.reshape(-1, 3)
 
 
 def plt_tables(lattice: int, ng: int = 64) -> Plt:

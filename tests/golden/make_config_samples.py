@@ -1,0 +1,154 @@
+"""Sampled oracle outputs of the full-size benchmark configurations.
+
+The CPU FP64 oracle (oracle/, the reference's schedule of 33 Orszag products)
+takes minutes at 256^3 and 512^3, too long to run inside a GPU test.  This
+script runs it once and keeps a fixed random sample of every output, so the
+device runs at full size can be compared against the oracle in seconds:
+
+  c2_sample.npz  256^3 3LPT sc, box 500 Mpc/h, z_start 24, seed 12345
+                 (ic_generator.cc:509-629, 931-1047): the six k-space
+                 potentials (growth factors applied) at 4096 modes, and the
+                 real-space displacement and velocity of each LPT order
+                 separately (Psi1, Psi2, Psi3 incl. the A3 curl, v1..v3) at
+                 4096 cells.
+  c4_sample.npz  three boxes of the 48-box 2LPT workload, one per size
+                 (boxes 0, 16, 32: 128^3, 192^3, 256^3 with their seeds and
+                 box sizes), the six real-space fields at 4096 cells each.
+  c3_sample.npz  512^3 3LPT + PLT, bcc (2 x 512^3 particles), box
+                 1000 Mpc/h, z_start 24, seed 12345: the six potentials at
+                 4096 modes and FP64 positions / velocities of 8192 particles
+                 (particle_generator.hh:499-544, 597-639).
+
+Sample indices come from numpy default_rng(20261203) and are stored with the
+values.  Peak memory is about 32 GB (512^3); run time is several minutes on
+8 cores.
+
+Usage: python tests/golden/make_config_samples.py [c2] [c3] [c4]
+"""
+import ctypes as C
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import oracle as O  # noqa: E402
+
+NMODES, NCELLS, NPART = 4096, 4096, 8192
+
+
+def _log(msg):
+    print(f"[{time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def _sample_modes(rng, n):
+    nc = n // 2 + 1
+    flat = np.sort(rng.choice(n * n * nc, size=NMODES, replace=False))
+    return flat, np.stack(np.unravel_index(flat, (n, n, nc)), axis=1).astype(np.int32)
+
+
+def _potentials(cfg, oc):
+    n = cfg.n
+    _log(f"noise {n}^3")
+    wn = O.ngenic_fill(cfg.seed, n)
+    _log("potentials (33 Orszag products)")
+    pots = O.potentials(cfg, oc, wn)
+    del wn
+    return pots
+
+
+def make_c2():
+    n, box, z, seed = 256, 500.0, 24.0, 12345
+    cfg = O.ic_cfg(n, box, 3, seed)
+    oc = O.baseline_cosmology(z)
+    coef = O.lpt_coefs(cfg, oc)
+    t0 = time.time()
+    phi, phi2, phi3, A3 = _potentials(cfg, oc)
+    rng = np.random.default_rng(20261203)
+    flat, modes = _sample_modes(rng, n)
+    pk = np.stack([O.kview(p).reshape(-1)[flat] for p in (phi, phi2, phi3, *A3)])
+    cells = np.sort(rng.choice(n ** 3, size=NCELLS, replace=False))
+    cidx = np.stack(np.unravel_index(cells, (n, n, n)), axis=1).astype(np.int32)
+    zero = O.empty_field(n)
+    per_order = {1: (phi, zero, zero, (zero,) * 3), 2: (zero, phi2, zero, (zero,) * 3),
+                 3: (zero, zero, phi3, A3)}
+    fields = np.zeros((3, 6, NCELLS))
+    for o, pots in per_order.items():
+        for comp in range(6):
+            f = O.component_k(cfg, coef, pots, comp)
+            O.fft_c2r(f)
+            fields[o - 1, comp] = O.rview(f)[cidx[:, 0], cidx[:, 1], cidx[:, 2]]
+        _log(f"order {o} fields")
+    np.savez_compressed(os.path.join(HERE, "c2_sample.npz"), n=n, box=box, zstart=z, seed=seed, modes=modes,
+                        potentials=pk, cells=cidx, fields=fields, seconds=time.time() - t0)
+    _log("c2_sample.npz written")
+
+
+def make_c3():
+    n, box, z, seed = 512, 1000.0, 24.0, 12345
+    cfg = O.ic_cfg(n, box, 3, seed, lattice=1, plt=1)
+    oc = O.baseline_cosmology(z)
+    coef = O.lpt_coefs(cfg, oc)
+    t0 = time.time()
+    pots = _potentials(cfg, oc)
+    rng = np.random.default_rng(20261203)
+    flat, modes = _sample_modes(rng, n)
+    phi, phi2, phi3, A3 = pots
+    pk = np.stack([O.kview(p).reshape(-1)[flat] for p in (phi, phi2, phi3, *A3)])
+    _log("PLT tables (bcc, 64^3)")
+    plt = O.plt_tables(1, 64)
+    npart = 2 * n ** 3
+    pos = np.zeros(3 * npart)
+    vel = np.zeros(3 * npart)
+    lib = O.lib()
+    for comp in range(6):
+        f = O.component_k(cfg, coef, pots, comp, plt=plt)
+        O.fft_c2r(f)
+        if comp < 3:
+            lib.orc_lattice_set_positions(n, box, 1, comp, cfg.lunit, f, pos)
+        else:
+            lib.orc_lattice_set_velocities(n, box, 1, comp - 3, f, vel)
+        _log(f"component {comp}")
+        del f
+    lib.orc_plt_free(C.byref(plt))
+    parts = np.sort(rng.choice(npart, size=NPART, replace=False))
+    P = pos.reshape(-1, 3)[parts]
+    V = vel.reshape(-1, 3)[parts]
+    np.savez_compressed(os.path.join(HERE, "c3_sample.npz"), n=n, box=box, zstart=z, seed=seed, modes=modes,
+                        potentials=pk, particles=parts.astype(np.int64), pos=P, vel=V,
+                        lunit=cfg.lunit, seconds=time.time() - t0)
+    _log("c3_sample.npz written")
+
+
+def make_c4():
+    from monofonic_amd.batch import batch_boxes  # the mt19937_64 box list (pinned by a known answer)
+    boxes = batch_boxes()
+    rng = np.random.default_rng(20261203)
+    out = {}
+    t0 = time.time()
+    for slot, i in enumerate((0, 16, 32)):
+        n, seed, box = boxes[i]
+        cfg = O.ic_cfg(n, box, 2, seed)
+        fields, _, _, _ = O.ic_run(cfg, O.baseline_cosmology(24.0))
+        cells = np.sort(rng.choice(n ** 3, size=NCELLS, replace=False))
+        cidx = np.stack(np.unravel_index(cells, (n, n, n)), axis=1).astype(np.int32)
+        out[f"box{slot}"] = np.array([i, n, seed], dtype=np.int64)
+        out[f"L{slot}"] = np.float64(box)
+        out[f"cells{slot}"] = cidx
+        out[f"fields{slot}"] = np.stack([O.rview(f)[cidx[:, 0], cidx[:, 1], cidx[:, 2]] for f in fields])
+        _log(f"box {i}: {n}^3 L={box:.3f}")
+    np.savez_compressed(os.path.join(HERE, "c4_sample.npz"), zstart=24.0, seconds=time.time() - t0, **out)
+    _log("c4_sample.npz written")
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["c2", "c3", "c4"]
+    if "c2" in which:
+        make_c2()
+    if "c3" in which:
+        make_c3()
+    if "c4" in which:
+        make_c4()

@@ -84,3 +84,30 @@ def test_argument_checks():
     assert lib.mfic_fft_size_supported(7) == 0
     assert lib.mfic_ngenic_seedtable(1, 8, None) == -1
     assert lib.mfic_cosmo_compute(None, None) == -1
+
+
+def test_context_closes_children_before_itself(monkeypatch):
+    """Context.close() closes every live handle owner it adopted before it
+    calls mfic_ctx_destroy (which refuses while handles are alive)"""
+    from monofonic_amd.grid import Context
+    calls = []
+    monkeypatch.setattr(L, "call", lambda name, *a: calls.append(name))
+
+    class Child:
+        def __init__(self, tag):
+            self.tag = tag
+
+        def close(self):
+            calls.append(self.tag)
+
+    ctx = Context.__new__(Context)
+    ctx.h = C.c_void_p(0x1000)
+    import weakref
+    ctx._children = weakref.WeakSet()
+    kids = [Child(f"child{i}") for i in range(3)]
+    for k in kids:
+        ctx.adopt(k)
+    ctx.close()
+    assert calls[-1] == "mfic_ctx_destroy"
+    assert sorted(calls[:-1]) == ["child0", "child1", "child2"]
+    assert not ctx.h

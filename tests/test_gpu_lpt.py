@@ -1,8 +1,9 @@
 """Device LPT path vs the CPU FP64 oracle on the same seeded white noise.
 
 FP64 device results must agree with the oracle to rel-L2 1e-10 (different FFT
-algorithms and summation orders); FP32 device results to rel-L2 2e-5 and
-max-abs relative error 5e-4 (tolerances for the FP32 bulk path).
+algorithms and summation orders; 5e-9 for the second- and third-order
+potentials, whose sources cancel more); FP32 device results to rel-L2 1e-5,
+the FP32 displacement target, for every field and potential.
 """
 import numpy as np
 import pytest
@@ -15,7 +16,7 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-TOL = {F64: 1e-10, F32: 2e-5}
+TOL = {F64: 1e-10, F32: 1e-5}
 
 
 def _k_grid(ctx, arr, prec, box):
@@ -64,7 +65,7 @@ def test_convolve2_matches_oracle(ctx, prec, kinds):
     conv.convolve2(lpt._operand(ga, kmap[kinds[0]][0], kinds[1]), lpt._operand(gb, kmap[kinds[2]][0], kinds[3]),
                    go, (L.OP_MADD, 0.75))
     got = go.download_k()
-    assert rel_l2(got - O.kview(init), O.kview(out_ref) - O.kview(init)) < 10 * TOL[prec]
+    assert rel_l2(got - O.kview(init), O.kview(out_ref) - O.kview(init)) < TOL[prec]
     conv.close()
 
 
@@ -81,7 +82,7 @@ def test_convolve3_matches_oracle(ctx, prec):
     conv = lpt.OrszagConvolver(ctx, (n,) * 3, (box,) * 3, prec)
     conv.convolve3(lpt._operand(g[0], L.HESS, (0, 1)), lpt._operand(g[1], L.HESS, (1, 1)),
                    lpt._operand(g[2], L.HESS, (2, 0)), go, L.OP_ASSIGN)
-    assert rel_l2(go.download_k(), O.kview(out_ref)) < 10 * TOL[prec]
+    assert rel_l2(go.download_k(), O.kview(out_ref)) < TOL[prec]
     conv.close()
 
 
@@ -108,7 +109,7 @@ def test_potentials_match_oracle(ctx, order, prec):
         got = run.potential(w).download_k() * scale[w]
         err = rel_l2(got, O.kview(refs[w]))
         print(f"potential {w} order {order} prec {prec}: rel-L2 {err:.3e}")
-        assert err < (50 if w > 0 else 1) * TOL[prec], (w, err)
+        assert err < (50 if w > 0 and prec == F64 else 1) * TOL[prec], (w, err)
     run.close()
 
 
@@ -147,7 +148,7 @@ def test_fields_match_oracle(ctx, order, prec):
         ref = O.rview(fields[c])
         err = rel_l2(got, ref)
         print(f"field {c} order {order} prec {prec}: rel-L2 {err:.3e}")
-        assert err < 50 * TOL[prec], (c, err)
+        assert err < (50 if prec == F64 else 1) * TOL[prec], (c, err)
     run.close()
 
 

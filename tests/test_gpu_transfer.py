@@ -14,7 +14,7 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-TOL = {F64: 1e-10, F32: 2e-5}
+TOL = {F64: 1e-10, F32: 1e-5}
 
 
 @pytest.mark.gpu
