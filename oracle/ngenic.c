@@ -150,3 +150,50 @@ void orc_ngenic_fill(const uint32_t *T, size_t n, double *kd) {
         }
     }
 }
+
+/* The reference's Gaussian transform evaluates std::log, std::cos and
+ * std::sin of the host libm (random_ngenic.cc:119-120); this library uses
+ * correctly rounded versions so that host and device agree bit for bit.
+ * Walks every draw of the non-distributed fill (the same column streams and
+ * skips as orc_ngenic_fill) and counts where the two differ:
+ *   out[0] draws stored, out[1] log differs, out[2] cos differs,
+ *   out[3] sin differs, out[4] the stored value (re or im) differs.
+ * The libm calls go through volatile pointers so the compiler neither
+ * folds nor fuses them. */
+void orc_ngenic_libm_compare(const uint32_t *T, size_t n, uint64_t out[5]) {
+    double (*volatile p_log)(double) = log;
+    double (*volatile p_cos)(double) = cos;
+    double (*volatile p_sin)(double) = sin;
+    const size_t nc = n / 2 + 1;
+    uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : c0, c1, c2, c3, c4)
+    for (long il = 0; il < (long)n; ++il) {
+        size_t i = (size_t)il;
+        orc_ranlxd r;
+        for (size_t j = 0; j < n; ++j) {
+            orc_ranlxd_set(&r, T[i * n + j], 202);
+            for (size_t k = 0; k < nc; ++k) {
+                double phase = orc_ranlxd_uniform(&r) * 2 * 3.14159265358979323846;
+                double ampl;
+                do {
+                    ampl = orc_ranlxd_uniform(&r);
+                } while (ampl == 0 || ampl == 1);
+                if (i == n / 2 || j == n / 2 || k == n / 2) continue;
+                if (i == 0 && j == 0 && k == 0) continue;
+                /* only draws the fill stores (k > 0, or half of the k = 0 plane) */
+                if (k == 0 && !((i == 0 && j < n / 2) || (i > 0 && i < n / 2))) continue;
+                const double lc = orc_cr_log(ampl), lg = p_log(ampl);
+                double sn, cs;
+                orc_cr_sincos(phase, &sn, &cs);
+                const double cg = p_cos(phase), sg = p_sin(phase);
+                const double a1 = sqrt(-lc), a2 = sqrt(-lg);
+                c0 += 1;
+                c1 += lc != lg;
+                c2 += cs != cg;
+                c3 += sn != sg;
+                c4 += (a1 * cs != a2 * cg) || (a1 * sn != a2 * sg);
+            }
+        }
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3; out[4] = c4;
+}

@@ -71,6 +71,10 @@ unsigned long orc_ranlxd_get(orc_ranlxd *st);    /* gsl_rng_get (32 bit)  */
  * RANLUX test values above, structural invariants and statistics. */
 void orc_ngenic_seedtable(long seed, size_t n, uint32_t *table);
 void orc_ngenic_fill(const uint32_t *table, size_t n, double *kdata);
+/* draw-by-draw comparison of the correctly rounded Gaussian transform with
+ * the host libm's log/cos/sin (what the reference calls); counts in out[5]:
+ * draws, log / cos / sin mismatches, stored-value mismatches */
+void orc_ngenic_libm_compare(const uint32_t *table, size_t n, uint64_t out[5]);
 
 /* ----------------------------------------------------------- cosmology --- */
 typedef struct {

@@ -100,6 +100,7 @@ def lib():
         L.orc_ranlxd_get.argtypes = [C.POINTER(Ranlxd)]
         L.orc_ngenic_seedtable.argtypes = [C.c_long, C.c_size_t, _u32p]
         L.orc_ngenic_fill.argtypes = [_u32p, C.c_size_t, _dp]
+        L.orc_ngenic_libm_compare.argtypes = [_u32p, C.c_size_t, C.POINTER(C.c_uint64 * 5)]
         L.orc_cosmo_defaults.argtypes = [C.POINTER(Cosmo)]
         L.orc_cosmo_init.argtypes = [C.POINTER(Cosmo)]
         L.orc_cosmo_init.restype = C.c_int
@@ -209,6 +210,14 @@ def ngenic_fill(seed: int, n: int) -> np.ndarray:
     out = empty_field(n)
     lib().orc_ngenic_fill(t, n, out)
     return out
+
+
+def ngenic_libm_compare(seed: int, n: int) -> dict:
+    """counts of draws whose host-libm Gaussian transform (the reference's
+    std::log/cos/sin) differs from the correctly rounded one"""
+    out = (C.c_uint64 * 5)()
+    lib().orc_ngenic_libm_compare(seedtable(seed, n), n, C.byref(out))
+    return dict(zip(("draws", "log", "cos", "sin", "value"), [int(x) for x in out]))
 
 
 def cosmology(transfer=None, **kw) -> Cosmo:

@@ -95,8 +95,12 @@ def _column_restated(seed_word, n):
             if ampl != 0.0 and ampl != 1.0:
                 break
         a = math.sqrt(-math.log(ampl))
-        out.append(complex(a * math.cos(phase), a * math.sin(phase)))
-    return np.array(out)
+        # glibc (math.*, what the reference's std:: calls reach) is correctly
+        # rounded for this draw when its log, cos and sin equal the oracle's
+        s, c = O.cr_sincos(phase)
+        agree = math.log(ampl) == O.cr_log(ampl) and math.cos(phase) == c and math.sin(phase) == s
+        out.append((complex(a * math.cos(phase), a * math.sin(phase)), agree))
+    return out
 
 
 def test_ngenic_columns_and_structure():
@@ -110,7 +114,11 @@ def test_ngenic_columns_and_structure():
         for kk in range(1, h):  # k > 0, off the Nyquist planes
             if i == h or j == h:
                 continue
-            assert abs(k[i, j, kk] - col[kk]) <= 4e-16 * abs(col[kk])
+            z, agree = col[kk]
+            if agree:
+                assert k[i, j, kk] == z  # bit for bit where glibc rounds correctly
+            else:
+                assert abs(k[i, j, kk] - z) <= 4e-16 * abs(z)
     # Nyquist planes and DC are zero
     assert np.all(k[h] == 0) and np.all(k[:, h] == 0) and np.all(k[:, :, h] == 0)
     assert k[0, 0, 0] == 0
@@ -133,3 +141,15 @@ def test_ngenic_statistics():
     ph = np.angle(z)
     hist, _ = np.histogram(ph, bins=8, range=(-np.pi, np.pi))
     assert hist.min() > 0.9 * hist.mean()
+
+
+def test_gaussian_draws_differ_from_glibc_only_where_glibc_misrounds():
+    """C1's 64^3 seed-12345 draws: the stored values differ from the host
+    libm evaluation (the reference's std::log/cos/sin on glibc) only in draws
+    where one of glibc's log, cos, sin is not correctly rounded, by one
+    rounding; the counts are those measured for glibc 2.35 (64^3: 386 of
+    125,023 draws; 512^3: 189,028 of 66,716,415, both 0.3%)"""
+    c = O.ngenic_libm_compare(12345, 64)
+    assert c["draws"] == 125023
+    assert c["value"] <= c["log"] + c["cos"] + c["sin"]
+    assert c["value"] < 0.005 * c["draws"]
