@@ -10,7 +10,8 @@ Prints one JSON line.  Extra objects: "roofline" (dominant kernel, HIP-event
 timing on the library stream during the timed steps), "cpu_baseline" (the
 CPU FP64 oracle on a bounded 256^3 sample of the same configuration),
 "fp64" (same workload with FP64 bulk), "accuracy" (FP32 vs FP64 device
-fields, rel-L2 / max error per component), "accuracy_by_order" (the 256^3
+runs of the benchmarked particle path: rel-L2 / max error of Psi = pos - q and
+v per component over all 2 x 512^3 particles), "accuracy_by_order" (the 256^3
 3LPT workload: FP32 vs FP64 error of Psi1, Psi2, Psi3 and the velocity
 separately), "batch" (48 independent 2LPT
 boxes of 128^3-256^3 replayed as one HIP graph with autotuned tiles) and
@@ -149,12 +150,25 @@ def cpu_baseline(n_sample=256):
     n_sample^3 with the same cell size; PLT table build excluded as on the GPU."""
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
     box = 1000.0 * n_sample / 512
     cfg = O.ic_cfg(n_sample, box, 3, 12345, lattice=1, plt=1)
     oc = O.baseline_cosmology(24.0)
     _, _, _, st = O.ic_run(cfg, oc, particles=True)
     t = st[4] - st[2]
     return {"value": round(n_sample ** 3 / t, 1), "unit": "cells/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail,
             "sample": f"oracle (C/OpenMP FP64, own FFT, reference schedule of 33 Orszag products) 3LPT+PLT bcc "
                       f"{n_sample}^3 box {box:g} Mpc/h z=24 (same cell size as 512^3/1000); PLT table build "
                       f"excluded; {t:.2f} s",
@@ -290,25 +304,56 @@ def adjoint_bench(n=256, box=500.0, reps=3, fd_cells=64):
             "fd_median_err_over_rms_grad": float(np.median(errs))}
 
 
+class _DeviceArray:
+    """__cuda_array_interface__ view of a library-owned device buffer, so
+    torch can read it in place"""
+
+    def __init__(self, ptr, count, typestr):
+        self.__cuda_array_interface__ = {"shape": (count,), "typestr": typestr, "data": (ptr, False), "version": 3}
+
+
 def accuracy(ctx, n, box):
-    """FP32 vs FP64 device component fields (same seed): rel-L2 and max error"""
-    import numpy as np
+    """The benchmarked path itself (bcc particles, FP64 q+Psi) in FP32 and in
+    FP64 on the device, same seed: rel-L2 and max error of each displacement
+    component Psi = pos - q and each velocity component of all 2 n^3
+    particles, computed on the device against the FP64 run"""
+    import torch
     from monofonic_amd import F32, F64, _lib as L
     from monofonic_amd.lpt import ICConfig, ICRun
-    res = {}
-    fields = {}
+    runs, arr = {}, {}
     for prec in (F64, F32):
         cfg = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=L.BCC, plt=True, prec=prec, seed=12345,
-                       particles=False)
+                       particles=True)
         r = ICRun(ctx, cfg)
         r.execute()
-        fields[prec] = [r.field(c).download(np.float64)[:, :, :n] for c in range(6)]
-        r.close()
+        pos, vel, npart = r.particles_device()
+        arr[prec] = [torch.as_tensor(_DeviceArray(p, 3 * npart, "<f8"), device="cuda").view(npart, 3)
+                     for p in (pos, vel)]
+        runs[prec] = r
+    torch.cuda.synchronize()
+    res = {}
+    n3 = n ** 3
+    ar = torch.arange(n, device="cuda", dtype=torch.float64)
     for c, name in enumerate(("dx", "dy", "dz", "vx", "vy", "vz")):
-        a, b = fields[F32][c], fields[F64][c]
-        d = a - b
-        res[name] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(b)),
-                     "max_abs_rel": float(np.abs(d).max() / np.abs(b).max())}
+        d = c % 3
+        num = den = 0.0
+        dmax = bmax = 0.0
+        for s in range(2):
+            sl = slice(s * n3, (s + 1) * n3)
+            a = arr[F32][c // 3][sl, d]
+            b = arr[F64][c // 3][sl, d]
+            if c < 3:
+                idx = ar.view(-1, 1, 1) if d == 0 else (ar.view(1, -1, 1) if d == 1 else ar.view(1, 1, -1))
+                q = ((idx + 0.5 * s) / n).expand(n, n, n).reshape(-1)
+                a, b = a - q, b - q
+            diff = a - b
+            num += float(torch.sum(diff * diff))
+            den += float(torch.sum(b * b))
+            dmax = max(dmax, float(diff.abs().max()))
+            bmax = max(bmax, float(b.abs().max()))
+        res[name] = {"rel_l2": (num / den) ** 0.5, "max_abs_rel": dmax / bmax}
+    for r in runs.values():
+        r.close()
     return res
 
 
@@ -331,7 +376,7 @@ def accuracy_by_order(ctx, n=256, box=500.0, z=24.0):
     kvec = (ki[:, None, None] * kf, ki[None, :, None] * kf, kz[None, None, :] * kf)
 
     def grad(f, d):
-        return np.fft.irfftn(1j * kvec[d] * f, s=(n, n, n), norm="ortho")
+        return np.fft.irfftn(1j * kvec[d] * f, s=(n, n, n), axes=(0, 1, 2), norm="ortho")
 
     psi = {}
     for prec in (F64, F32):
@@ -478,7 +523,7 @@ def main():
                         "fp32_speedup": round(ms64 / ms, 3), "roofline": roofline(rep64, max(2, args.steps // 2))}
         log(f"FP64: {ms64:.2f} ms/step")
     if not args.no_accuracy:
-        log("accuracy: FP32 vs FP64 device fields")
+        log("accuracy: FP32 vs FP64 device particles (Psi = pos - q, v)")
         line["accuracy"] = {"reference": "FP64 device run (same seed, same schedule)", **accuracy(ctx, n, box)}
         log("accuracy by LPT order: 256^3 3LPT FP32 vs FP64")
         line["accuracy_by_order"] = {"workload": "3LPT sc 256^3 (384^3 padded), box 500 Mpc/h, z_start 24, seed 12345",

@@ -240,7 +240,7 @@ __device__ __forceinline__ bool float_rounding_settled(double y) {
 // one value of the white-noise field from a (phase, amplitude) draw pair
 template <typename T>
 __device__ __forceinline__ cx<T> ngenic_value(double phase, double ampl, const CrTables& tabs) {
-    double zr, zi;
+    double zr = 0.0, zi = 0.0;
     bool done = false;
     if constexpr (sizeof(T) == 4) {
         const double a = sqrt(-log(ampl));
