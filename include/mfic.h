@@ -174,6 +174,19 @@ int mfic_cosmo_eval(const mfic_cosmo_params *p, const char *what, const double *
  * same size (phi may equal wn). */
 int mfic_phi1_from_noise(mfic_grid phi, mfic_grid wn, const mfic_cosmo_params *p, int fixing,
                          int inversion);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mfic_phi1_from_noise with the same kernels and results,
+ * but its per-|k| amplitude table lives in a fixed device array of the
+ * library (442,369 doubles, 3.4 MiB per GPU, kept until the process ends)
+ * instead of context scratch, so the call allocates nothing.  Limits: cubic
+ * grids up to 768^3 (larger ones return MFIC_ERR_SIZE; non-cubic grids need
+ * no table); Eisenstein & Hu only (tk_n > 0 returns MFIC_ERR_UNSUPPORTED,
+ * since a tabulated T(k) needs its spline nodes copied to the device).  All
+ * calls on one GPU share the array: calls that overlap on one GPU (different
+ * streams or host threads) race; calls in one stream's order do not. */
+int mfic_phi1_from_noise_noalloc(mfic_grid phi, mfic_grid wn, const mfic_cosmo_params *p, int fixing,
+                                 int inversion);
 
 /* ------------------------------------------------------------ convolver --- */
 typedef struct {

@@ -100,6 +100,7 @@ SIGNATURES = {
     "mfic_cosmo_compute": (C.c_int, [C.POINTER(CosmoParams), C.POINTER(CosmoScalars)]),
     "mfic_cosmo_eval": (C.c_int, [C.POINTER(CosmoParams), C.c_char_p, _P, _P, C.c_size_t]),
     "mfic_phi1_from_noise": (C.c_int, [_P, _P, C.POINTER(CosmoParams), C.c_int, C.c_int]),
+    "mfic_phi1_from_noise_noalloc": (C.c_int, [_P, _P, C.POINTER(CosmoParams), C.c_int, C.c_int]),
     "mfic_conv_create": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_double), C.c_int, C.POINTER(_P)]),
     "mfic_conv_destroy": (C.c_int, [_P]),
     "mfic_conv_convolve2": (C.c_int, [_P, C.POINTER(Operand), C.POINTER(Operand), _P, C.c_int, C.c_double]),

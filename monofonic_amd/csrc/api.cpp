@@ -101,6 +101,13 @@ int mfic_ctx_create(int device, mfic_ctx* out) {
             delete c;
             throw Error(MFIC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
         }
+        try {
+            phi1_scratch_setup(device);  // device is current
+        } catch (...) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            throw;
+        }
         *out = c;
     });
 }

@@ -1,6 +1,10 @@
 // Pointwise grid kernels: scaling, precision conversion, DC zeroing, inverse
 // Laplacian, Fourier shift and the phi1 construction (white noise -> linear
 // displacement potential with the E&H amplitude in FP64).
+#include <cmath>
+#include <mutex>
+#include <string>
+
 #include "cosmology.hpp"
 #include "fft_core.hpp"
 #include "kspace.hpp"
@@ -195,7 +199,8 @@ void grid_phase_shift(mfic_grid g, const double s[3]) {
     MFIC_LAUNCH_CHECK();
 }
 
-void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double volfac, int fixing, int inversion) {
+void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double volfac, int fixing, int inversion,
+                     double* table) {
     MFIC_CHECK(eh.tk_n == 0 || eh.tk != nullptr, MFIC_ERR_STATE, "tabulated transfer function not on the device");
     const KGeom G = geom_of(phi);
     const size_t nc = (size_t)G.n0 * G.n1 * G.nc;
@@ -203,7 +208,8 @@ void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double vol
     ProfScope prof(phi->ctx, "phi1", 2.0 * phi->bytes());
     if (G.n0 == G.n1 && G.n1 == G.n2 && G.kf0 == G.kf1 && G.kf1 == G.kf2) {
         const int nmax = 3 * (G.n0 / 2) * (G.n0 / 2);
-        double* tab = static_cast<double*>(ctx_scratch(phi->ctx, "phi1_tab", (size_t)(nmax + 1) * sizeof(double)));
+        double* tab = table ? table
+                            : static_cast<double*>(ctx_scratch(phi->ctx, "phi1_tab", (size_t)(nmax + 1) * sizeof(double)));
         hipLaunchKernelGGL(k_phi1_table, dim3(nblocks((size_t)nmax + 1, bs)), dim3(bs), 0, phi->ctx->stream, tab, nmax,
                            G.kf0, eh);
         MFIC_LAUNCH_CHECK();
@@ -225,4 +231,67 @@ void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double vol
     MFIC_LAUNCH_CHECK();
 }
 
+// --------------------------------------------- no-allocation phi1 entry --
+// Scratch of mfic_phi1_from_noise_noalloc: the FP64 -A(k)/k^2 table per
+// integer |n|^2 of a cubic grid, 3 (N/2)^2 + 1 values, sized for every grid
+// the library transforms (N <= 768): 442,369 doubles (3.4 MiB).  Each GPU
+// the library runs on holds its own copy until the process ends.  In host
+// code `scratch` is only a stand-in for the symbol: its device address comes
+// from hipGetSymbolAddress, once per device with that device current (at
+// context creation), and only that address reaches kernels.
+constexpr size_t kPhi1ScratchCount = 3 * 384 * 384 + 1;
+static __device__ __align__(256) double scratch[kPhi1ScratchCount];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+void phi1_scratch_setup(int device) {
+    MFIC_CHECK(device >= 0 && device < kMaxDevices, MFIC_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw Error(MFIC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<double*>(p);
+}
+
+static double* phi1_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    MFIC_CHECK(device >= 0 && device < kMaxDevices && g_scratch_dev[device], MFIC_ERR_STATE,
+               "no-allocation scratch not set up on this device (create a context first)");
+    return g_scratch_dev[device];
+}
+
 }  // namespace mfic
+
+extern "C" int mfic_phi1_from_noise_noalloc(mfic_grid phi, mfic_grid wn, const mfic_cosmo_params* p, int fixing,
+                                            int inversion) {
+    using namespace mfic;
+    return guard_call([&] {
+        MFIC_CHECK(phi && phi->d && wn && wn->d, MFIC_ERR_ARG, "null or unallocated grid");
+        MFIC_CHECK(p, MFIC_ERR_ARG, "null cosmology");
+        MFIC_CHECK(phi->n[0] == wn->n[0] && phi->n[1] == wn->n[1] && phi->n[2] == wn->n[2] && phi->prec == wn->prec,
+                   MFIC_ERR_ARG, "phi and noise grids differ in size or precision");
+        MFIC_CHECK(wn->space == MFIC_KSPACE, MFIC_ERR_STATE, "white noise must be in k-space");
+        MFIC_CHECK(p->tk_n == 0, MFIC_ERR_UNSUPPORTED,
+                   "the no-allocation phi1 takes the Eisenstein & Hu transfer function only (a tabulated one needs "
+                   "its spline nodes copied to device memory)");
+        const KGeom G = geom_of(phi);
+        const bool cubic = G.n0 == G.n1 && G.n1 == G.n2 && G.kf0 == G.kf1 && G.kf1 == G.kf2;
+        if (cubic) {
+            const size_t need = 3 * (size_t)(G.n0 / 2) * (size_t)(G.n0 / 2) + 1;
+            MFIC_CHECK(need <= kPhi1ScratchCount, MFIC_ERR_SIZE,
+                       "grid of " + std::to_string(G.n0) + "^3 needs a " + std::to_string(need) +
+                           "-entry phi1 table; the no-allocation scratch holds " + std::to_string(kPhi1ScratchCount) +
+                           " (N <= 768)");
+        }
+        EhParams eh;
+        mfic_cosmo_scalars s;
+        cosmo_compute(*p, &s, &eh);
+        const double volfac = std::pow(phi->L[0] / (double)phi->n[0] / 2.0 / 3.14159265358979323846, 1.5);
+        phi1_from_noise(phi, wn, eh, volfac, fixing, inversion, cubic ? phi1_scratch(phi->ctx->device) : nullptr);
+        phi->space = MFIC_KSPACE;
+    });
+}

@@ -10,5 +10,10 @@ void grid_zero_dc(mfic_grid g);
 void grid_inverse_laplacian(mfic_grid g);
 void grid_phase_shift(mfic_grid g, const double s[3]);
 void convert_buffer(mfic_ctx ctx, const void* in, int in_prec, void* out, int out_prec, size_t n);
-void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double volfac, int fixing, int inversion);
+// table: device buffer of 3 (N/2)^2 + 1 doubles for the cubic-grid path, or
+// null to use the context's scratch
+void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double volfac, int fixing, int inversion,
+                     double* table = nullptr);
+// resolves the no-allocation scratch's address on `device` (current)
+void phi1_scratch_setup(int device);
 }  // namespace mfic
