@@ -34,7 +34,8 @@ enum mfic_status {
     MFIC_ERR_ALLOC = -4,       /* device allocation failed                      */
     MFIC_ERR_STATE = -5,       /* wrong space / unallocated grid                */
     MFIC_ERR_CONFIG = -6,      /* invalid configuration value                  */
-    MFIC_ERR_UNSUPPORTED = -7  /* feature outside the hot path                  */
+    MFIC_ERR_UNSUPPORTED = -7, /* feature outside the hot path                  */
+    MFIC_ERR_IO = -8           /* file output failed (HDF5 missing or failing)  */
 };
 enum mfic_prec { MFIC_F32 = 0, MFIC_F64 = 1 };
 /* same numbering as the reference's space_t (grid_fft.hh:29) */
@@ -116,7 +117,9 @@ int mfic_fft_forward(mfic_grid g);
 int mfic_fft_backward(mfic_grid g);
 int mfic_fft_size_supported(size_t n);
 /* autotuner hook: override the tile (pencils per block for kind 0 strided
- * passes, rows per block for kind 1 contiguous-axis passes) and the LDS pitch
+ * passes and for kind 2, the bcc particle path's axis-0 pass that also forms
+ * the shifted sub-lattice; rows per block for kind 1 contiguous-axis
+ * passes) and the LDS pitch
  * padding for one transform length and precision; tile <= 0 restores the
  * built-in choice.  No reference counterpart (FFTW planning,
  * grid_fft.cc:53-54, plays this role there). */
@@ -339,6 +342,38 @@ int mfic_lpt2_adjoint(mfic_lpt2 p, mfic_grid noise, mfic_grid cot_x, mfic_grid c
  * framework tensor of shape [n0][n1][n2+2]); destroying it frees nothing */
 int mfic_grid_wrap(mfic_ctx ctx, const size_t n[3], const double L[3], int prec, int space, void *device_ptr,
                    mfic_grid *out);
+
+/* ------------------------------------------------------- particle output --- */
+/* The Gadget-HDF5 output plugin (output_gadget_hdf5.cc:36-318, the
+ * gadget_hdf5 and gadget_hdf5_double plugins; [output] keys in the options)
+ * with HDF_IO.hh's datasets and attributes:
+ *   /PartType1/Coordinates, /PartType1/Velocities  N x 3, 32-bit floats (the
+ *     reference's vector writer, HDF_IO.hh:850-885, creates them as
+ *     H5T_NATIVE_FLOAT whatever the plugin's precision);
+ *   /PartType1/ParticleIDs  uint32, or uint64 with long_ids; particle p of an
+ *     n^3 lattice with 2^lattice sub-lattices gets 2^lattice * (p mod n^3) +
+ *     p / n^3 (particle_generator.hh:229-251);
+ *   /Header (output_gadget_hdf5.cc:175-210) and /ICs_parameters.
+ * The HDF5 C library is loaded at run time (MFIC_HDF5_LIB, libhdf5.so, or
+ * the conda copy); without it the calls return MFIC_ERR_UNSUPPORTED. */
+typedef struct {
+    int double_precision; /* gadget_hdf5_double (has_64bit_reals)                  */
+    int long_ids;         /* [output] UseLongids                                   */
+    int gadget2_compat;   /* [output] Gadget2Compatibility: 32-bit NumPart arrays  */
+} mfic_gadget_opts;
+/* output_plugin::position_unit / velocity_unit / mass_unit of the writer
+ * (output_gadget_hdf5.cc:79-85): BoxLength, BoxLength / sqrt(a_start) and
+ * rho_crit BoxLength^3 [1e10 Msun/h]; a run written with mfic_gadget_write
+ * must use the first two as lunit and vunit */
+int mfic_gadget_units(double boxlength, double zstart, double *lunit, double *vunit, double *munit);
+/* write_particle_data + the header of a run with particles = 1 (sc or bcc),
+ * streamed chunk by chunk from its device arrays (the run must have
+ * executed); cosmological header values come from the run's parameters */
+int mfic_gadget_write(mfic_run r, const char *fname, const mfic_gadget_opts *opts);
+/* the same file from host arrays: FP64 [p][3] positions and velocities of
+ * the 2^lattice * n^3 particles of cfg (cfg->lunit / vunit as above) */
+int mfic_gadget_write_host(const char *fname, const mfic_ic_config *cfg, const mfic_cosmo_params *cosmo,
+                           const mfic_gadget_opts *opts, const double *pos, const double *vel);
 
 #ifdef __cplusplus
 }

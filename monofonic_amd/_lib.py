@@ -20,7 +20,7 @@ SC, BCC, FCC = 0, 1, 2
 
 ERRORS = {
     -1: "MFIC_ERR_ARG", -2: "MFIC_ERR_SIZE", -3: "MFIC_ERR_HIP", -4: "MFIC_ERR_ALLOC",
-    -5: "MFIC_ERR_STATE", -6: "MFIC_ERR_CONFIG", -7: "MFIC_ERR_UNSUPPORTED",
+    -5: "MFIC_ERR_STATE", -6: "MFIC_ERR_CONFIG", -7: "MFIC_ERR_UNSUPPORTED", -8: "MFIC_ERR_IO",
 }
 
 
@@ -68,6 +68,10 @@ _sz3 = C.c_size_t * 3
 _d3 = C.c_double * 3
 
 # name -> (restype, argtypes); every int-returning entry point is status-checked
+class GadgetOpts(C.Structure):
+    _fields_ = [("double_precision", C.c_int), ("long_ids", C.c_int), ("gadget2_compat", C.c_int)]
+
+
 SIGNATURES = {
     "mfic_last_error": (C.c_char_p, []),
     "mfic_version": (C.c_char_p, []),
@@ -115,6 +119,11 @@ SIGNATURES = {
     "mfic_lpt_component": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(LptCoef), _P, C.c_double] + [_P] * 7),
     "mfic_lattice_readout": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, _P]),
     "mfic_run_create": (C.c_int, [_P, C.POINTER(IcConfig), C.POINTER(CosmoParams), C.POINTER(_P)]),
+    "mfic_gadget_units": (C.c_int, [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
+    "mfic_gadget_write": (C.c_int, [_P, C.c_char_p, C.POINTER(GadgetOpts)]),
+    "mfic_gadget_write_host": (C.c_int, [C.c_char_p, C.POINTER(IcConfig), C.POINTER(CosmoParams),
+                                         C.POINTER(GadgetOpts), _P, _P]),
     "mfic_run_destroy": (C.c_int, [_P]),
     "mfic_run_execute": (C.c_int, [_P, C.c_int]),
     "mfic_run_field": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),

@@ -96,6 +96,10 @@ int mfic_ctx_create(int device, mfic_ctx* out) {
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lds <= 0)
             lds = 65536;
         c->max_lds = (size_t)lds;
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
+            ncu = 256;
+        c->num_cu = ncu;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete c;
@@ -359,7 +363,8 @@ int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, in
     return guard([&] {
         MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
         MFIC_CHECK(fft_length_supported(length), MFIC_ERR_SIZE, "unsupported FFT length");
-        MFIC_CHECK(kind == 0 || kind == 1, MFIC_ERR_ARG, "kind must be 0 (strided pencils) or 1 (rows)");
+        MFIC_CHECK(kind >= 0 && kind <= 2, MFIC_ERR_ARG,
+                   "kind must be 0 (strided pencils), 1 (rows) or 2 (axis-0 pass with the shifted sub-lattice)");
         MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
         const auto key = std::make_tuple(length, kind, prec);
         if (tile <= 0) { ctx->tiles.erase(key); return; }

@@ -11,6 +11,7 @@
 #include "kspace.hpp"
 #include "lpt.hpp"
 #include "mfic_internal.hpp"
+#include "output.hpp"
 #include "ngenic.hpp"
 #include "orszag.hpp"
 
@@ -433,6 +434,57 @@ int mfic_run_potential(mfic_run r, int which, mfic_grid* out) {
         MFIC_CHECK(r && out && which >= 0 && which < 6, MFIC_ERR_ARG, "bad argument");
         MFIC_CHECK(r->pot[which] != nullptr, MFIC_ERR_STATE, "potential not computed for this LPT order");
         *out = r->pot[which];
+    });
+}
+
+int mfic_gadget_units(double boxlength, double zstart, double* lunit, double* vunit, double* munit) {
+    return guard_call([&] {
+        MFIC_CHECK(lunit && vunit && munit, MFIC_ERR_ARG, "null output");
+        MFIC_CHECK(boxlength > 0 && zstart > -1.0, MFIC_ERR_CONFIG, "bad box length or redshift");
+        gadget_units(boxlength, zstart, lunit, vunit, munit);
+    });
+}
+
+static void check_gadget_units(const mfic_ic_config& c, double zstart) {
+    double lu, vu, mu;
+    gadget_units(c.L, zstart, &lu, &vu, &mu);
+    MFIC_CHECK(std::fabs(c.lunit - lu) <= 1e-12 * lu && std::fabs(c.vunit - vu) <= 1e-12 * vu, MFIC_ERR_CONFIG,
+               "particles must be in the Gadget-HDF5 units: lunit = " + std::to_string(lu) +
+                   ", vunit = " + std::to_string(vu) + " (mfic_gadget_units)");
+}
+
+int mfic_gadget_write(mfic_run r, const char* fname, const mfic_gadget_opts* opts) {
+    return guard_call([&] {
+        MFIC_CHECK(r && fname && opts, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(r->cfg.particles && r->d_pos && r->d_vel, MFIC_ERR_STATE, "run was configured for field output");
+        check_gadget_units(r->cfg, r->cosmo.zstart);
+        ParticleSource src;
+        src.ctx = r->ctx;
+        src.pos = r->d_pos;
+        src.vel = r->d_vel;
+        src.npart = r->npart;
+        src.ncell = r->cfg.n * r->cfg.n * r->cfg.n;
+        src.overload = 1u << r->cfg.lattice;
+        MFIC_HIP(hipSetDevice(r->ctx->device));
+        gadget_write(fname, r->cfg, r->cosmo, *opts, src, r->d_tk != nullptr);
+    });
+}
+
+int mfic_gadget_write_host(const char* fname, const mfic_ic_config* cfg, const mfic_cosmo_params* cosmo,
+                           const mfic_gadget_opts* opts, const double* pos, const double* vel) {
+    return guard_call([&] {
+        MFIC_CHECK(fname && cfg && cosmo && opts && pos && vel, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(cfg->lattice == MFIC_LATTICE_SC || cfg->lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
+                   "ParticleLoad must be sc or bcc");
+        MFIC_CHECK(cfg->n > 0, MFIC_ERR_SIZE, "empty lattice");
+        check_gadget_units(*cfg, cosmo->zstart);
+        ParticleSource src;
+        src.pos = pos;
+        src.vel = vel;
+        src.ncell = cfg->n * cfg->n * cfg->n;
+        src.overload = 1u << cfg->lattice;
+        src.npart = src.ncell * src.overload;
+        gadget_write(fname, *cfg, *cosmo, *opts, src, cosmo->tk_n > 0);
     });
 }
 

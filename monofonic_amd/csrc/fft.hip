@@ -404,9 +404,14 @@ template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], dou
 template <typename T, int L>
 static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc) {
     constexpr int P = Plan<L>::P;
-    int tk = 64;
+    // at most 16 pencils per tile by default: at 512^3 FP32 the 16-pencil
+    // tile takes 0.46 ms per launch, the 32-pencil tile 1.37 ms (two outputs
+    // per pencil double the registers of the 1024-thread workgroups)
+    int tk = 16;
     while (tk > 4 && ((size_t)L * tk * pencil_lds_elem<T, true>() > pencil_lds_cap<true>() || P * tk > 1024)) tk /= 2;
-    const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    // its own tuning key (kind 2): this pass keeps two outputs per pencil, so
+    // a tile tuned for the plain pencils (kind 0) does not carry over
+    const auto ov = ctx->tiles.find(std::make_tuple(L, 2, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) tk = ov->second.first;
     while (tk > 1 && ((size_t)L * tk * pencil_lds_elem<T, true>() > max_lds(ctx) || P * tk > 1024)) tk /= 2;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));

@@ -62,6 +62,7 @@ struct mfic_ctx_s {
     // scratch buffers owned by the context (reused across calls)
     std::map<std::string, std::pair<void*, size_t>> scratch;
     size_t max_lds = 0;  // LDS bytes per workgroup (device attribute)
+    int num_cu = 0;      // compute units of the device
     // tile overrides from the autotuner: (length, kind, prec) -> (tile, lds pad)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tiles;
     // stream capture into a HIP graph in progress (mfic_graph_begin): no

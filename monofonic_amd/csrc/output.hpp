@@ -1,0 +1,26 @@
+// Particle output writers (output.hip).
+#pragma once
+#include <cstddef>
+
+#include "mfic_internal.hpp"
+
+namespace mfic {
+
+// where the particle values come from: the device arrays of a run, staged
+// chunk by chunk on ctx's stream, or host arrays (ctx = null); FP64 [p][3]
+// positions and velocities of npart = overload * ncell particles
+struct ParticleSource {
+    mfic_ctx ctx = nullptr;
+    const double *pos = nullptr, *vel = nullptr;
+    size_t npart = 0, ncell = 0;
+    unsigned overload = 1;
+};
+
+// output_plugin units of the Gadget-HDF5 writer (output_gadget_hdf5.cc:79-85)
+void gadget_units(double boxlength, double zstart, double* lunit, double* vunit, double* munit);
+// one Gadget-HDF5 file of a run's particles; tabulated: the run used a
+// tabulated transfer function (recorded in ICs_parameters)
+void gadget_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_params& cosmo,
+                  const mfic_gadget_opts& opts, const ParticleSource& src, bool tabulated);
+
+}  // namespace mfic

@@ -14,6 +14,9 @@ device runs at full size can be compared against the oracle in seconds:
   c4_sample.npz  three boxes of the 48-box 2LPT workload, one per size
                  (boxes 0, 16, 32: 128^3, 192^3, 256^3 with their seeds and
                  box sizes), the six real-space fields at 4096 cells each.
+  p384_sample.npz  384^3 (576^3 padded) 3LPT sc, box 750 Mpc/h, z_start 24,
+                 seed 99: the six potentials at 4096 modes (the padded row
+                 length whose transforms take 32 lanes).
   c3_sample.npz  512^3 3LPT + PLT, bcc (2 x 512^3 particles), box
                  1000 Mpc/h, z_start 24, seed 12345: the six potentials at
                  4096 modes and FP64 positions / velocities of 8192 particles
@@ -23,7 +26,7 @@ Sample indices come from numpy default_rng(20261203) and are stored with the
 values.  Peak memory is about 32 GB (512^3); run time is several minutes on
 8 cores.
 
-Usage: python tests/golden/make_config_samples.py [c2] [c3] [c4]
+Usage: python tests/golden/make_config_samples.py [c2] [c3] [c4] [p384]
 """
 import ctypes as C
 import os
@@ -144,6 +147,18 @@ def make_c4():
     _log("c4_sample.npz written")
 
 
+def make_p384():
+    n, box, z, seed = 384, 750.0, 24.0, 99
+    cfg = O.ic_cfg(n, box, 3, seed)
+    t0 = time.time()
+    phi, phi2, phi3, A3 = _potentials(cfg, O.baseline_cosmology(z))
+    flat, modes = _sample_modes(np.random.default_rng(20261203), n)
+    pk = np.stack([O.kview(p).reshape(-1)[flat] for p in (phi, phi2, phi3, *A3)])
+    np.savez_compressed(os.path.join(HERE, "p384_sample.npz"), n=n, box=box, zstart=z, seed=seed, modes=modes,
+                        potentials=pk, seconds=time.time() - t0)
+    _log("p384_sample.npz written")
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["c2", "c3", "c4"]
     if "c2" in which:
@@ -152,3 +167,5 @@ if __name__ == "__main__":
         make_c3()
     if "c4" in which:
         make_c4()
+    if "p384" in which:
+        make_p384()
