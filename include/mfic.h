@@ -343,6 +343,25 @@ int mfic_lpt2_adjoint(mfic_lpt2 p, mfic_grid noise, mfic_grid cot_x, mfic_grid c
 int mfic_grid_wrap(mfic_ctx ctx, const size_t n[3], const double L[3], int prec, int space, void *device_ptr,
                    mfic_grid *out);
 
+/* --------------------------------------------- complex grids and PPT --- */
+/* Grid_FFT<ccomplex_t> transforms (grid_fft.cc:56-63, 207-270) on a device
+ * array [n0][n1][n2] of complex values (FP32 or FP64 pairs): in place,
+ * unitary (1/sqrt(n0 n1 n2) in both directions), forward e^{-ikx}; every n
+ * = 2^a 3^b with 4 <= n <= 768. */
+int mfic_fft_c2c(mfic_ctx ctx, void *data, const size_t n[3], int prec, int forward);
+/* Semiclassical perturbation theory fields of the field_eulerian output
+ * (ic_generator.cc:786-897) from an executed run's potentials: hbar =
+ * 1.01 max|d phi| / (pi D+) over neighbouring cells (the reference's loop,
+ * which leaves out the last x-plane), psi = exp(i (phi + phi2) / (hbar D+))
+ * (phi alone for LPTorder 1), one drift exp(-i hbar k^2 D+ / 2) through the
+ * c2c transforms, the kick exp(i phi2 / (hbar D+)), delta = |psi|^2 - 1 and
+ * v_d = D+ vfac / L * velocity_unit * hbar Im(psi* d_d psi) / (1 + delta)
+ * with d_d = i k_d in k-space (get_k, Nyquist at +n/2).  The run's species
+ * is dark matter without baryons (C_species = 0, so sqrt(rho) = 1).  Outputs
+ * are real-space grids of the run's size and precision; hbar may be NULL. */
+int mfic_ppt_fields(mfic_run r, double velocity_unit, mfic_grid delta, mfic_grid vx, mfic_grid vy, mfic_grid vz,
+                    double *hbar);
+
 /* ------------------------------------------------------- particle output --- */
 /* The Gadget-HDF5 output plugin (output_gadget_hdf5.cc:36-318, the
  * gadget_hdf5 and gadget_hdf5_double plugins; [output] keys in the options)

@@ -119,6 +119,8 @@ SIGNATURES = {
     "mfic_lpt_component": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(LptCoef), _P, C.c_double] + [_P] * 7),
     "mfic_lattice_readout": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, _P]),
     "mfic_run_create": (C.c_int, [_P, C.POINTER(IcConfig), C.POINTER(CosmoParams), C.POINTER(_P)]),
+    "mfic_fft_c2c": (C.c_int, [_P, _P, C.POINTER(C.c_size_t), C.c_int, C.c_int]),
+    "mfic_ppt_fields": (C.c_int, [_P, C.c_double, _P, _P, _P, _P, C.POINTER(C.c_double)]),
     "mfic_gadget_units": (C.c_int, [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
     "mfic_gadget_write": (C.c_int, [_P, C.c_char_p, C.POINTER(GadgetOpts)]),

@@ -12,6 +12,7 @@
 #include "lpt.hpp"
 #include "mfic_internal.hpp"
 #include "output.hpp"
+#include "ppt.hpp"
 #include "ngenic.hpp"
 #include "orszag.hpp"
 
@@ -485,6 +486,69 @@ int mfic_gadget_write_host(const char* fname, const mfic_ic_config* cfg, const m
         src.overload = 1u << cfg->lattice;
         src.npart = src.ncell * src.overload;
         gadget_write(fname, *cfg, *cosmo, *opts, src, cosmo->tk_n > 0);
+    });
+}
+
+int mfic_fft_c2c(mfic_ctx ctx, void* data, const size_t n[3], int prec, int forward) {
+    return guard_call([&] {
+        MFIC_CHECK(ctx && data && n, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
+        const double norm = 1.0 / std::sqrt((double)n[0] * (double)n[1] * (double)n[2]);
+        if (prec == MFIC_F64) fft_c2c<double>(ctx, data, n, forward != 0, norm);
+        else fft_c2c<float>(ctx, data, n, forward != 0, norm);
+    });
+}
+
+int mfic_ppt_fields(mfic_run r, double velocity_unit, mfic_grid delta, mfic_grid vx, mfic_grid vy, mfic_grid vz,
+                    double* hbar) {
+    return guard_call([&] {
+        MFIC_CHECK(r, MFIC_ERR_ARG, "null run");
+        const mfic_ic_config& c = r->cfg;
+        mfic_grid outs[4] = {delta, vx, vy, vz};
+        for (mfic_grid g : outs) {
+            need(g, "output");
+            MFIC_CHECK(g->prec == c.prec && g->n[0] == c.n && g->n[1] == c.n && g->n[2] == c.n, MFIC_ERR_ARG,
+                       "PPT outputs must be real grids of the run's size and precision");
+        }
+        MFIC_CHECK(r->pot[0] && r->pot[0]->space == MFIC_KSPACE, MFIC_ERR_STATE, "execute the run first");
+        const size_t n = c.n, es = c.prec == MFIC_F64 ? 8 : 4;
+        const size_t nn[3] = {n, n, n};
+        const double norm = 1.0 / std::sqrt((double)n * n * n);
+        // growth-scaled real-space potentials (ic_generator.cc:644-658, 809, 847)
+        struct Buf {
+            void* p = nullptr;
+            ~Buf() {
+                if (p) (void)hipFree(p);
+            }
+        } ph1, ph2, psi, grad;
+        const size_t greal = n * n * (n + 2) * es;
+        ph1.p = dmalloc(greal, "PPT phi");
+        if (c.lpt_order >= 2) ph2.p = dmalloc(greal, "PPT phi2");
+        psi.p = dmalloc(n * n * n * 2 * es, "PPT psi");
+        grad.p = dmalloc(n * n * n * 2 * es, "PPT grad psi");
+        MFIC_HIP(hipMemcpyAsync(ph1.p, r->pot[0]->d, greal, hipMemcpyDeviceToDevice, r->ctx->stream));
+        if (ph2.p) MFIC_HIP(hipMemcpyAsync(ph2.p, r->pot[1]->d, greal, hipMemcpyDeviceToDevice, r->ctx->stream));
+        if (c.prec == MFIC_F64) {
+            fft_c2r<double>(r->ctx, (double*)ph1.p, nn, norm * r->coef.g1);
+            if (ph2.p) fft_c2r<double>(r->ctx, (double*)ph2.p, nn, norm * r->coef.g2);
+        } else {
+            fft_c2r<float>(r->ctx, (float*)ph1.p, nn, norm * r->coef.g1);
+            if (ph2.p) fft_c2r<float>(r->ctx, (float*)ph2.p, nn, norm * r->coef.g2);
+        }
+        PptInput in;
+        in.n = (int)n;
+        in.order = c.lpt_order;
+        in.L = c.L;
+        in.Dplus0 = r->coef.Dplus0;
+        in.vunit = r->coef.Dplus0 * r->coef.vfac / c.L * velocity_unit;
+        in.phi_r = ph1.p;
+        in.phi2_r = ph2.p;
+        in.psi = psi.p;
+        in.grad = grad.p;
+        void* vel[3] = {vx->d, vy->d, vz->d};
+        ppt_fields(r->ctx, c.prec, in, delta->d, vel, hbar);
+        MFIC_HIP(hipStreamSynchronize(r->ctx->stream));
+        for (mfic_grid g : outs) g->space = MFIC_RSPACE;
     });
 }
 

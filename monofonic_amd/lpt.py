@@ -300,3 +300,20 @@ def _hip():
         _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _HIP.hipMemcpy.restype = C.c_int
     return _HIP
+
+
+def fft_c2c(ctx: Context, data_ptr: int, shape: Sequence[int], prec: int, forward: bool = True) -> None:
+    """Grid_FFT<ccomplex_t> transform (grid_fft.cc:56-63) of a device
+    [n0][n1][n2] complex array in place: unitary, forward e^{-ikx}"""
+    L.call("mfic_fft_c2c", ctx.h, C.c_void_p(data_ptr), (C.c_size_t * 3)(*shape), prec, int(forward))
+
+
+def ppt_fields(run: ICRun, velocity_unit: float = 1.0):
+    """Semiclassical PT density and velocity of the field_eulerian output
+    (ic_generator.cc:786-897) from an executed run: ([delta, vx, vy, vz] as
+    real-space Grids of the run's size and precision, hbar)"""
+    n, box = run.cfg.n, run.cfg.box
+    outs = [Grid(run.ctx, (n,) * 3, (box,) * 3, prec=run.cfg.prec) for _ in range(4)]
+    hb = C.c_double()
+    L.call("mfic_ppt_fields", run.h, velocity_unit, *[g.h for g in outs], C.byref(hb))
+    return outs, hb.value

@@ -202,3 +202,19 @@ void orc_fft3d_c2r(double *d, int n0, int n1, int n2) {
         free(z);
     }
 }
+
+/* Grid_FFT<ccomplex_t> transform (grid_fft.cc:56-63, 207-270) of a complex
+ * [n0][n1][n2] array, in place, unitary, sign -1 forward / +1 backward.
+ * pinned: numpy.fft.fftn / ifftn(norm="ortho"). */
+void orc_fft3d_c2c(double *d, int n0, int n1, int n2, int sign) {
+    get_plan(n2);
+    const long rows = (long)n0 * n1;
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < rows; ++r) orc_fft1d(d + 2 * r * n2, n2, sign);
+    pencils(d, n0, n1, n2, 1, sign);
+    pencils(d, n0, n1, n2, 0, sign);
+    const double norm = 1.0 / sqrt((double)n0 * (double)n1 * (double)n2);
+    const long tot = 2L * n0 * n1 * n2;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < tot; ++i) d[i] *= norm;
+}

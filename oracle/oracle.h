@@ -35,6 +35,10 @@ struct orc_plt;
  * pinned: numpy.fft.rfftn/irfftn(norm="ortho") fixtures + identities. */
 void orc_fft3d_r2c(double *data, int n0, int n1, int n2);
 void orc_fft3d_c2r(double *data, int n0, int n1, int n2);
+/* Grid_FFT<ccomplex_t> transform (grid_fft.cc:56-63): complex [n0][n1][n2],
+ * in place, unitary, sign -1 forward / +1 backward.
+ * pinned: numpy.fft.fftn / ifftn(norm="ortho"). */
+void orc_fft3d_c2c(double *data, int n0, int n1, int n2, int sign);
 /* 1D complex FFT of length n (interleaved re/im), sign -1 forward, +1 back,
  * unnormalised.  pinned: numpy.fft.fft fixtures. */
 void orc_fft1d(double *z, int n, int sign);
@@ -195,6 +199,15 @@ void orc_scale(int n, double *f, double s);
  * each an n x n x (n+2) real buffer, or particle output when pos != NULL. */
 int orc_ic_run(const orc_ic_cfg *cfg, orc_cosmo *c, double *fields6,
                double *pos, double *vel, double stage_seconds[8]);
+
+/* ----------------------------------------------------------------- PPT --- */
+/* Semiclassical PT fields (ic_generator.cc:786-897) from the growth-scaled
+ * k-space potentials (phi2 may be NULL for LPTorder 1): delta, vx, vy, vz as
+ * real n x n x (n+2) buffers; returns 0.  pinned: unitarity (mean delta = 0),
+ * the drift's momentum invariant, and the linear limit (Zel'dovich density
+ * and velocity on large scales) in tests/test_oracle_ppt.py. */
+int orc_ppt_fields(const orc_ic_cfg *cfg, const orc_lpt_coef *co, const double *phi_k, const double *phi2_k,
+                   double velocity_unit, double *delta, double *vx, double *vy, double *vz, double *hbar);
 
 /* ----------------------------------------------------------------- PLT --- */
 /* particle::lattice_gradient (particle_plt.hh:50-587) tables on an ng^3 mesh */

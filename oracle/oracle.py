@@ -90,6 +90,10 @@ def lib():
         L.orc_fft3d_r2c.argtypes = [_dp, C.c_int, C.c_int, C.c_int]
         L.orc_fft3d_c2r.argtypes = [_dp, C.c_int, C.c_int, C.c_int]
         L.orc_fft1d.argtypes = [_dp, C.c_int, C.c_int]
+        L.orc_fft3d_c2c.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_ppt_fields.argtypes = [C.POINTER(IcCfg), C.POINTER(LptCoef), _dp, C.c_void_p, C.c_double, _dp, _dp,
+                                     _dp, _dp, C.POINTER(C.c_double)]
+        L.orc_ppt_fields.restype = C.c_int
         L.orc_cr_log.restype = C.c_double
         L.orc_cr_log.argtypes = [C.c_double]
         L.orc_cr_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -175,6 +179,25 @@ def fft1d(z: np.ndarray, sign: int) -> np.ndarray:
     buf = np.ascontiguousarray(z, dtype=np.complex128).copy()
     lib().orc_fft1d(buf.view(np.float64), buf.size, sign)
     return buf
+
+
+def fft_c2c(z: np.ndarray, sign: int) -> np.ndarray:
+    """unitary 3D c2c transform of a complex (n0, n1, n2) array (copy)"""
+    buf = np.ascontiguousarray(z, dtype=np.complex128).copy()
+    n0, n1, n2 = buf.shape
+    lib().orc_fft3d_c2c(buf.view(np.float64).reshape(-1), n0, n1, n2, sign)
+    return buf
+
+
+def ppt_fields(cfg, coef, phi_k: np.ndarray, phi2_k, velocity_unit: float):
+    """semiclassical PT fields: (delta, vx, vy, vz) real (n, n, n+2) arrays and hbar"""
+    n = cfg.n
+    outs = [empty_field(n) for _ in range(4)]
+    hb = C.c_double()
+    rc = lib().orc_ppt_fields(C.byref(cfg), C.byref(coef), phi_k, _ptr(phi2_k), velocity_unit, *outs, C.byref(hb))
+    if rc != 0:
+        raise RuntimeError("oracle PPT failed")
+    return outs, hb.value
 
 
 def cr_log(x: float) -> float:

@@ -206,3 +206,24 @@ def test_c4_full_graph_bit_identical_to_eager(ctx):
             _flush(f"C4 graph replay {rep}: 48 boxes x 6 fields bit-identical")
     finally:
         b.close()
+
+
+# ------------------------------------------------------ 384^3 (576 padded) --
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_384_potentials_match_oracle(ctx, prec):
+    """the padded row length whose transforms take 32 lanes (576^3 padded),
+    against a sampled oracle run (tests/golden/p384_sample.npz)"""
+    s = load_sample("p384_sample.npz")
+    n, box, z, seed = int(s["n"]), float(s["box"]), float(s["zstart"]), int(s["seed"])
+    _flush(f"384^3 3LPT potentials prec={prec}")
+    cos = lpt.baseline_cosmology(z)
+    coef = lpt.lpt_coefficients(cos, n, box, 3)
+    run = lpt.ICRun(ctx, lpt.ICConfig(n=n, box=box, zstart=z, lpt_order=3, prec=prec, seed=seed), cos)
+    run.execute()
+    scale = [coef.g1, coef.g2, coef.g3, coef.g3c, coef.g3c, coef.g3c]
+    for w in range(6):
+        err = rel_l2(sample_k(run.potential(w), s["modes"]) * scale[w], s["potentials"][w])
+        _flush(f"384^3 prec {prec} potential {w}: rel-L2 {err:.2e}")
+        assert err < pot_tol(prec, w), (w, err)
+    run.close()
