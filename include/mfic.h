@@ -165,7 +165,11 @@ typedef struct {
 } mfic_cosmo_scalars;
 void mfic_cosmo_defaults(mfic_cosmo_params *p);
 int mfic_cosmo_compute(const mfic_cosmo_params *p, mfic_cosmo_scalars *out);
-/* host evaluations for checks: D+(a), f(a), H(a), E&H T(k) [k in h/Mpc] */
+/* host evaluations, what = "D" D+(a), "f" f(a), "H" H(a), "vfact" f a H / h,
+ * "T" T(k) and "A" k^{n_s/2} T(k) sqrt(pnorm) [k in h/Mpc], T from the table
+ * when tk_n > 0, else E&H.  "T" with E&H costs only the fit; the others first
+ * integrate the growth tables and the sigma_8 normalisation (milliseconds), so
+ * pass many x per call. */
 int mfic_cosmo_eval(const mfic_cosmo_params *p, const char *what, const double *x, double *y,
                     size_t count);
 

@@ -363,14 +363,21 @@ void cosmo_compute(const mfic_cosmo_params& p, mfic_cosmo_scalars* s, EhParams* 
 }
 
 void cosmo_eval(const mfic_cosmo_params& p, const std::string& what, const double* x, double* y, size_t n) {
+    if (what == "T" && p.tk_n == 0) {
+        // the E&H fit needs only its constants, not the growth tables and the
+        // sigma_8 normalisation (a tabulated T(k) takes the full set-up below)
+        const EhParams eh = eh_setup(p);
+        for (size_t i = 0; i < n; ++i) y[i] = eh_transfer(eh, x[i]);
+        return;
+    }
     Calc c;
     c.p = p;
     c.run();
     for (size_t i = 0; i < n; ++i) {
-        if (what == "D") y[i] = c.Draw(x[i]) / c.s.Dnow;
+        if (what == "T") y[i] = eh_transfer(c.eh, x[i]);
+        else if (what == "D") y[i] = c.Draw(x[i]) / c.s.Dnow;
         else if (what == "f") y[i] = c.f(x[i]);
         else if (what == "H") y[i] = c.H(x[i]);
-        else if (what == "T") y[i] = eh_transfer(c.eh, x[i]);
         else if (what == "A") y[i] = std::pow(x[i], 0.5 * p.n_s) * eh_transfer(c.eh, x[i]) * c.s.sqrtpnorm;
         else if (what == "vfact") y[i] = c.f(x[i]) * x[i] * c.H(x[i]) / c.s.h;
         else throw std::invalid_argument("unknown quantity " + what);
