@@ -488,13 +488,16 @@ def main():
 
     n, box = args.n, 1000.0 * args.n / 512
     ctx = Context(0)
-    if not args.no_tune:
-        from monofonic_amd.batch import autotune
-        log(f"autotune tiles for {n}^3 3LPT FP32 (plan time)")
-        tuned_ms = autotune(ctx, n, F32, 3)
-    log(f"plan: 3LPT+PLT bcc n={n} box={box} FP32")
     cfg = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=BCC, plt=True, prec=F32, seed=12345,
                    particles=True)
+    if not args.no_tune:
+        from monofonic_amd.batch import autotune_config, chosen_plans
+        from monofonic_amd.lpt import baseline_cosmology
+        log(f"autotune radix plans and tiles for the {n}^3 3LPT+PLT bcc FP32 run (plan time)")
+        tuned_ms = autotune_config(ctx, cfg, baseline_cosmology(24.0))
+        plans = {f"L{k[0]}_pass{k[1]}": v for k, v in chosen_plans(ctx, n, F32).items()}
+        log(f"tuned: {tuned_ms:.2f} ms per execution, non-default plans {plans}")
+    log(f"plan: 3LPT+PLT bcc n={n} box={box} FP32")
     run = ICRun(ctx, cfg)
     log(f"timing {args.steps} steps after {args.warmup} warmup")
     ms, rep = time_steps(ctx, run, args.steps, args.warmup, profile=True)
@@ -508,7 +511,7 @@ def main():
         "config": {"workload": f"3LPT+PLT bcc {n}^3 ({3 * n // 2}^3 padded), box {box:g} Mpc/h, z_start 24, "
                                "FP32 bulk, FP64 q+Psi", "n": n, "padded": 3 * n // 2, "box": box, "zstart": 24.0,
                    "lpt_order": 3, "lattice": "bcc", "plt": True, "particles": 2 * n ** 3},
-        "autotune": None if args.no_tune else {"tuned_3lpt_fields_ms": round(tuned_ms, 3)},
+        "autotune": None if args.no_tune else {"tuned_ms": round(tuned_ms, 3), "plans": plans},
         "roofline": roofline(rep, args.steps),
         "kernels": kernel_table(rep, args.steps),
     }

@@ -125,6 +125,33 @@ int mfic_fft_size_supported(size_t n);
  * grid_fft.cc:53-54, plays this role there). */
 int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, int lds_pad);
 
+/* Radix plans.  Each transform length has a compiled default plan (elements
+ * per thread and the radix of each Stockham stage); the padded-pass lengths
+ * 512 and 768 also have an alternative with twice the elements per thread
+ * and 16 threads per transform.  A strided pass of one kind selects its plan
+ * per length and precision (the row passes keep the default).  No reference
+ * counterpart: FFTW picks its codelets while planning (grid_fft.cc:53-54). */
+enum mfic_fft_pass {
+    MFIC_PASS_PENCILS = 0,    /* strided passes of N-grid transforms (axes 0 and 1) */
+    MFIC_PASS_AX0_SHIFT = 2,  /* axis-0 c2r pass that also forms the shifted bcc sub-lattice */
+    MFIC_PASS_PAD_C2R_A = 3,  /* convolver padded passes (length 3N/2): axis-0 c2r */
+    MFIC_PASS_PAD_C2R_B = 4,  /*   axis-1 c2r */
+    MFIC_PASS_PAD_R2C_A = 5,  /*   axis-0 r2c */
+    MFIC_PASS_PAD_R2C_B = 6,  /*   axis-1 r2c */
+    MFIC_PASS_TRUNC_AX0 = 7,  /*   two-stage truncation filter, axis 0 */
+    MFIC_PASS_TRUNC_AX1 = 8   /*   two-stage truncation filter, axis 1 */
+};
+/* number of compiled radix plans for a length (0: unsupported length) */
+int mfic_fft_plan_count(int length);
+/* elements per thread and radices (0 after the last stage) of plan
+ * `variant` of a length */
+int mfic_fft_plan_info(int length, int variant, int *elems_per_thread, int radices[4]);
+/* select plan `variant` for one pass kind, length and precision on a
+ * context; variant < 0 restores the default.  mfic_fft_get_plan reports
+ * the current choice. */
+int mfic_fft_set_plan(mfic_ctx ctx, int length, int pass, int prec, int variant);
+int mfic_fft_get_plan(mfic_ctx ctx, int length, int pass, int prec, int *variant);
+
 /* --------------------------------------------------------------- noise --- */
 /* RNG_ngenic constructor seed table (random_ngenic.cc:39-75), host only:
  * table has n*n entries. */
@@ -315,13 +342,19 @@ int mfic_graph_begin(mfic_ctx ctx);
 int mfic_graph_end(mfic_ctx ctx, mfic_graph *out);
 int mfic_graph_launch(mfic_graph g, int blocking);
 int mfic_graph_destroy(mfic_graph g);
-/* Tile autotuner: for a box of size n, precision prec and LPT order, times
- * whole pipeline executions over the pencils-per-block choices of the N-grid
- * and padded strided passes and the rows-per-block choices of the N-grid
- * contiguous pass, keeping the fastest per transform length in the context
- * (mfic_fft_set_tile overrides).  best_ms (may be NULL) receives the tuned
+/* Autotuner: times whole pipeline executions of a temporary run while it
+ * walks, one knob at a time, the radix plan of every strided pass kind that
+ * has more than one (mfic_fft_set_plan), the pencils-per-block choices of the
+ * N-grid and padded strided passes and of the shifted sub-lattice pass, and
+ * the rows-per-block choices of the N-grid contiguous pass
+ * (mfic_fft_set_tile), then the plans once more; the winners stay in the
+ * context.  mfic_autotune tunes an n^3 box of precision prec and LPT order
+ * (sc lattice, component fields); mfic_autotune_config tunes the given run
+ * configuration and cosmology.  best_ms (may be NULL) receives the tuned
  * time of one execution. */
 int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double *best_ms);
+int mfic_autotune_config(mfic_ctx ctx, const mfic_ic_config *cfg, const mfic_cosmo_params *cosmo,
+                         double *best_ms);
 
 /* --------------------------------------------------- differentiable 2LPT --- */
 /* No reference counterpart (monofonIC has no derivative of its IC map).

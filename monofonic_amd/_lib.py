@@ -17,6 +17,11 @@ KSPACE, RSPACE = 0, 1
 OP_ASSIGN, OP_ADD, OP_SUB, OP_MADD = 0, 1, 2, 3
 HESS, SUM_HESS, DIFF_HESS, GRAD, IDENT = 0, 1, 2, 3, 4
 SC, BCC, FCC = 0, 1, 2
+# strided pass kinds with a selectable radix plan (mfic_fft_pass)
+PASS_PENCILS, PASS_AX0_SHIFT = 0, 2
+PASS_PAD_C2R_A, PASS_PAD_C2R_B, PASS_PAD_R2C_A, PASS_PAD_R2C_B, PASS_TRUNC_AX0, PASS_TRUNC_AX1 = 3, 4, 5, 6, 7, 8
+PASSES = (PASS_PENCILS, PASS_AX0_SHIFT, PASS_PAD_C2R_A, PASS_PAD_C2R_B, PASS_PAD_R2C_A, PASS_PAD_R2C_B,
+          PASS_TRUNC_AX0, PASS_TRUNC_AX1)
 
 ERRORS = {
     -1: "MFIC_ERR_ARG", -2: "MFIC_ERR_SIZE", -3: "MFIC_ERR_HIP", -4: "MFIC_ERR_ALLOC",
@@ -98,6 +103,10 @@ SIGNATURES = {
     "mfic_fft_backward": (C.c_int, [_P]),
     "mfic_fft_size_supported": (C.c_int, [C.c_size_t]),
     "mfic_fft_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mfic_fft_plan_count": (C.c_int, [C.c_int]),
+    "mfic_fft_plan_info": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "mfic_fft_set_plan": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mfic_fft_get_plan": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "mfic_ngenic_seedtable": (C.c_int, [C.c_long, C.c_size_t, _P]),
     "mfic_ngenic_fill": (C.c_int, [_P, C.c_long]),
     "mfic_cosmo_defaults": (None, [C.POINTER(CosmoParams)]),
@@ -136,6 +145,7 @@ SIGNATURES = {
     "mfic_graph_launch": (C.c_int, [_P, C.c_int]),
     "mfic_graph_destroy": (C.c_int, [_P]),
     "mfic_autotune": (C.c_int, [_P, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "mfic_autotune_config": (C.c_int, [_P, C.POINTER(IcConfig), C.POINTER(CosmoParams), C.POINTER(C.c_double)]),
     "mfic_lpt2_create": (C.c_int, [_P, C.c_size_t, C.c_double, C.c_int, C.POINTER(CosmoParams), C.c_double,
                                    C.POINTER(_P)]),
     "mfic_lpt2_destroy": (C.c_int, [_P]),

@@ -112,10 +112,57 @@ def capture(ctx: Context, fn) -> Graph:
 
 
 def autotune(ctx: Context, n: int, prec: int, lpt_order: int) -> float:
-    """tile autotuner for boxes of size n; returns the tuned time (ms)"""
+    """radix plan and tile autotuner for boxes of size n (sc lattice,
+    component fields); returns the tuned time (ms)"""
     ms = C.c_double()
     L.call("mfic_autotune", ctx.h, n, prec, lpt_order, C.byref(ms))
     return ms.value
+
+
+def autotune_config(ctx: Context, cfg, cosmo: L.CosmoParams) -> float:
+    """the autotuner on a given run configuration (lpt.ICConfig) and
+    cosmology; returns the tuned time (ms)"""
+    ms = C.c_double()
+    c = cfg.to_c()
+    L.call("mfic_autotune_config", ctx.h, C.byref(c), C.byref(cosmo), C.byref(ms))
+    return ms.value
+
+
+def plan_count(length: int) -> int:
+    """number of compiled radix plans of a transform length"""
+    return L.load().mfic_fft_plan_count(length)
+
+
+def plan_info(length: int, variant: int):
+    """(elements per thread, radices) of radix plan `variant` of a length"""
+    e = C.c_int()
+    r = (C.c_int * 4)()
+    L.call("mfic_fft_plan_info", length, variant, C.byref(e), r)
+    return e.value, tuple(x for x in r if x)
+
+
+def set_plan(ctx: Context, length: int, pass_kind: int, prec: int, variant: int) -> None:
+    """select radix plan `variant` (< 0: the default) for one strided pass kind"""
+    L.call("mfic_fft_set_plan", ctx.h, length, pass_kind, prec, variant)
+
+
+def get_plan(ctx: Context, length: int, pass_kind: int, prec: int) -> int:
+    v = C.c_int()
+    L.call("mfic_fft_get_plan", ctx.h, length, pass_kind, prec, C.byref(v))
+    return v.value
+
+
+def chosen_plans(ctx: Context, n: int, prec: int) -> dict:
+    """{(length, pass kind): variant} of every non-default plan a box of
+    size n can use"""
+    out = {}
+    for length in (n, 3 * n // 2):
+        if plan_count(length) > 1:
+            for p in L.PASSES:
+                v = get_plan(ctx, length, p, prec)
+                if v:
+                    out[(length, p)] = v
+    return out
 
 
 class Batch:

@@ -96,10 +96,6 @@ int mfic_ctx_create(int device, mfic_ctx* out) {
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lds <= 0)
             lds = 65536;
         c->max_lds = (size_t)lds;
-        int ncu = 0;
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
-            ncu = 256;
-        c->num_cu = ncu;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete c;
@@ -371,6 +367,43 @@ int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, in
         MFIC_CHECK((tile & (tile - 1)) == 0 && tile <= 64, MFIC_ERR_ARG, "tile must be a power of two <= 64");
         MFIC_CHECK(lds_pad >= 0 && lds_pad <= 8, MFIC_ERR_ARG, "lds_pad must be in [0, 8]");
         ctx->tiles[key] = {tile, lds_pad};
+    });
+}
+
+int mfic_fft_plan_count(int length) { return length > 0 ? fft_plan_count(length) : 0; }
+
+int mfic_fft_plan_info(int length, int variant, int* elems_per_thread, int radices[4]) {
+    return guard([&] {
+        MFIC_CHECK(elems_per_thread && radices, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(length > 0 && fft_length_supported(length), MFIC_ERR_SIZE, "unsupported FFT length");
+        fft_plan_info(length, variant, elems_per_thread, radices);
+    });
+}
+
+static void check_plan_key(mfic_ctx ctx, int length, int pass, int prec) {
+    MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
+    MFIC_CHECK(length > 0 && fft_length_supported(length), MFIC_ERR_SIZE, "unsupported FFT length");
+    MFIC_CHECK(pass == MFIC_PASS_PENCILS || (pass >= MFIC_PASS_AX0_SHIFT && pass <= MFIC_PASS_TRUNC_AX1), MFIC_ERR_ARG,
+               "pass must be one of mfic_fft_pass (row passes have one plan)");
+    MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
+}
+
+int mfic_fft_set_plan(mfic_ctx ctx, int length, int pass, int prec, int variant) {
+    return guard([&] {
+        check_plan_key(ctx, length, pass, prec);
+        const auto key = std::make_tuple(length, pass, prec);
+        if (variant <= 0) { ctx->plans.erase(key); return; }
+        MFIC_CHECK(variant < fft_plan_count(length), MFIC_ERR_ARG,
+                   "length " + std::to_string(length) + " has " + std::to_string(fft_plan_count(length)) + " radix plan(s)");
+        ctx->plans[key] = variant;
+    });
+}
+
+int mfic_fft_get_plan(mfic_ctx ctx, int length, int pass, int prec, int* variant) {
+    return guard([&] {
+        check_plan_key(ctx, length, pass, prec);
+        MFIC_CHECK(variant, MFIC_ERR_ARG, "null argument");
+        *variant = plan_variant(ctx, length, pass, prec);
     });
 }
 

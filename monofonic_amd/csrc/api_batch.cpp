@@ -1,6 +1,6 @@
 // C ABI, batching: HIP-graph capture of any sequence of library calls on a
 // context stream (many boxes / seeds replayed by one graph launch) and the
-// tile autotuner.  No reference counterpart: monofonIC runs one box per
+// tile and radix plan autotuner.  No reference counterpart: monofonIC runs one box per
 // process and leaves transform planning to FFTW (grid_fft.cc:53-54).
 #include <algorithm>
 #include <cmath>
@@ -72,16 +72,119 @@ int mfic_graph_destroy(mfic_graph g) {
     });
 }
 
-// Coordinate descent over the tile of each transform length a box of size n
-// uses (pencils per block for the strided N-grid and padded passes, rows per
-// block for the contiguous N-grid pass), timing whole pipeline executions of
-// a temporary run with HIP events.  Leaves the winners in the context.
+namespace {
+
+// Coordinate descent over the radix plan of every strided pass kind and the
+// tile of every transform length a run configuration uses (pencils per
+// block for the strided N-grid and padded passes and for the axis-0 pass of
+// the shifted sub-lattice, rows per block for the contiguous N-grid pass),
+// timing whole executions of a temporary run of that configuration with HIP
+// events: plans, then tiles, then the plans again under the chosen tiles.
+// Leaves the winners in the context and returns the best time.
+double autotune_config(mfic_ctx ctx, const mfic_ic_config& cfg, const mfic_cosmo_params& cp) {
+    MFIC_CHECK(!ctx->capturing, MFIC_ERR_STATE, "cannot autotune while capturing");
+    const int n = (int)cfg.n, prec = cfg.prec;
+    mfic_run run = nullptr;
+    // a failing call's own status code is passed on (a size or
+    // configuration error is not a HIP error)
+    auto ck = [](int rc) {
+        if (rc != MFIC_OK) throw mfic::Error(rc, mfic_last_error());
+    };
+    ck(mfic_run_create(ctx, &cfg, &cp, &run));
+    hipEvent_t a = nullptr, b = nullptr;
+    auto release = [&] {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        (void)mfic_run_destroy(run);
+    };
+    double result = 0.0;
+    try {
+        MFIC_HIP(hipEventCreate(&a));
+        MFIC_HIP(hipEventCreate(&b));
+        auto time_once = [&]() {
+            float best = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                MFIC_HIP(hipEventRecord(a, ctx->stream));
+                ck(mfic_run_execute(run, 0));
+                MFIC_HIP(hipEventRecord(b, ctx->stream));
+                MFIC_HIP(hipEventSynchronize(b));
+                float ms = 0.f;
+                MFIC_HIP(hipEventElapsedTime(&ms, a, b));
+                best = std::min(best, ms);
+            }
+            return best;
+        };
+        const bool shift = cfg.particles && cfg.lattice == MFIC_LATTICE_BCC;
+        const int np = 3 * n / 2;
+        // plan knobs: (length, pass) with more than one compiled plan
+        std::vector<std::pair<int, int>> plan_knobs;
+        if (mfic::fft_plan_count(n) > 1) {
+            plan_knobs.push_back({n, MFIC_PASS_PENCILS});
+            if (shift) plan_knobs.push_back({n, MFIC_PASS_AX0_SHIFT});
+        }
+        if (cfg.lpt_order > 1 && mfic::fft_plan_count(np) > 1)
+            for (int pass = MFIC_PASS_PAD_C2R_A; pass <= MFIC_PASS_TRUNC_AX1; ++pass) plan_knobs.push_back({np, pass});
+        struct Knob { int length, kind; std::vector<int> cand; };
+        std::vector<Knob> tile_knobs = {{n, 0, {4, 8, 16, 32, 64}}, {n / 2, 1, {1, 2, 4, 8, 16}}};
+        if (shift) tile_knobs.push_back({n, 2, {4, 8, 16, 32}});
+        if (cfg.lpt_order > 1) tile_knobs.push_back({np, 0, {4, 8, 16, 32, 64}});
+
+        ck(mfic_run_execute(run, 1));
+        float cur = time_once();
+        auto tune_plans = [&] {
+            for (const auto& k : plan_knobs) {
+                const auto key = std::make_tuple(k.first, k.second, prec);
+                const int had = mfic::plan_variant(ctx, k.first, k.second, prec);
+                int best_v = had;
+                float best = cur;
+                for (int v = 0; v < mfic::fft_plan_count(k.first); ++v) {
+                    if (v == had) continue;
+                    if (v) ctx->plans[key] = v;
+                    else ctx->plans.erase(key);
+                    const float t = time_once();
+                    if (t < best) { best = t; best_v = v; }
+                }
+                if (best_v) ctx->plans[key] = best_v;
+                else ctx->plans.erase(key);
+                cur = best;
+            }
+        };
+        tune_plans();
+        for (const auto& k : tile_knobs) {
+            const auto key = std::make_tuple(k.length, k.kind, prec);
+            const auto had = ctx->tiles.find(key);
+            std::pair<int, int> best_tile = had != ctx->tiles.end() ? had->second : std::make_pair(0, 0);
+            float best = cur;
+            for (int c : k.cand) {
+                ctx->tiles[key] = {c, 0};
+                const float t = time_once();
+                if (t < best) { best = t; best_tile = {c, 0}; }
+            }
+            if (best_tile.first > 0) ctx->tiles[key] = best_tile;
+            else ctx->tiles.erase(key);
+            cur = best;
+        }
+        tune_plans();
+        result = cur;
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        release();
+        throw;
+    }
+    release();
+    return result;
+}
+
+}  // namespace
+
+// The pipeline of an n^3 box of the given order and precision (sc lattice,
+// component fields, the benchmark cosmology).
 int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double* best_ms) {
     return mfic::guard_call([&] {
         MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
-        MFIC_CHECK(!ctx->capturing, MFIC_ERR_STATE, "cannot autotune while capturing");
         MFIC_CHECK(mfic_fft_size_supported(n) && n % 2 == 0, MFIC_ERR_SIZE, "unsupported grid size");
         MFIC_CHECK(lpt_order >= 1 && lpt_order <= 3, MFIC_ERR_CONFIG, "LPT order must be 1..3");
+        MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
         mfic_cosmo_params cp;
         mfic_cosmo_defaults(&cp);
         cp.Omega_m = 0.3;
@@ -99,61 +202,16 @@ int mfic_autotune(mfic_ctx ctx, size_t n, int prec, int lpt_order, double* best_
         cfg.prec = prec;
         cfg.seed = 1;
         cfg.lunit = cfg.vunit = 1.0;
-        mfic_run run = nullptr;
-        // a failing call's own status code is passed on (a size or
-        // configuration error is not a HIP error)
-        auto ck = [](int rc) {
-            if (rc != MFIC_OK) throw mfic::Error(rc, mfic_last_error());
-        };
-        ck(mfic_run_create(ctx, &cfg, &cp, &run));
-        hipEvent_t a, b;
-        MFIC_HIP(hipEventCreate(&a));
-        MFIC_HIP(hipEventCreate(&b));
-        auto time_once = [&]() {
-            float best = 1e30f;
-            for (int rep = 0; rep < 3; ++rep) {
-                MFIC_HIP(hipEventRecord(a, ctx->stream));
-                ck(mfic_run_execute(run, 0));
-                MFIC_HIP(hipEventRecord(b, ctx->stream));
-                MFIC_HIP(hipEventSynchronize(b));
-                float ms = 0.f;
-                MFIC_HIP(hipEventElapsedTime(&ms, a, b));
-                best = std::min(best, ms);
-            }
-            return best;
-        };
-        double result = 0.0;
-        try {
-            ck(mfic_run_execute(run, 1));
-            struct Knob { int length, kind; std::vector<int> cand; };
-            std::vector<Knob> knobs = {{(int)n, 0, {4, 8, 16, 32, 64}}, {(int)n / 2, 1, {1, 2, 4, 8, 16}}};
-            if (lpt_order > 1) knobs.push_back({(int)(3 * n / 2), 0, {4, 8, 16, 32, 64}});
-            float cur = time_once();
-            for (const auto& k : knobs) {
-                const auto key = std::make_tuple(k.length, k.kind, prec);
-                const auto had = ctx->tiles.find(key);
-                std::pair<int, int> best_tile = had != ctx->tiles.end() ? had->second : std::make_pair(0, 0);
-                float best = cur;
-                for (int c : k.cand) {
-                    ctx->tiles[key] = {c, 0};
-                    const float t = time_once();
-                    if (t < best) { best = t; best_tile = {c, 0}; }
-                }
-                if (best_tile.first > 0) ctx->tiles[key] = best_tile;
-                else ctx->tiles.erase(key);
-                cur = best;
-            }
-            result = cur;
-        } catch (...) {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            (void)mfic_run_destroy(run);
-            throw;
-        }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        ck(mfic_run_destroy(run));
-        if (best_ms) *best_ms = result;
+        const double ms = autotune_config(ctx, cfg, cp);
+        if (best_ms) *best_ms = ms;
+    });
+}
+
+int mfic_autotune_config(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_params* cosmo, double* best_ms) {
+    return mfic::guard_call([&] {
+        MFIC_CHECK(ctx && cfg && cosmo, MFIC_ERR_ARG, "null argument");
+        const double ms = autotune_config(ctx, *cfg, *cosmo);
+        if (best_ms) *best_ms = ms;
     });
 }
 

@@ -19,11 +19,11 @@ namespace mfic {
 // Transforms of length L along a strided axis.  Block = TK neighbouring
 // pencils (contiguous t) x P threads per pencil; a wavefront reads TK
 // consecutive complex values per pencil position.
-template <typename T, int L, int TK, bool FWD>
-__global__ void __launch_bounds__(Plan<L>::P * TK)
+template <typename T, int L, int TK, bool FWD, int V = 0>
+__global__ void __launch_bounds__((Plan<L, V>::P * TK))
 k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
           const cx<T>* __restrict__ tw, int tkp, int gx, long ntiles, int xcd) {
-    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+    constexpr int E = Plan<L, V>::E, P = Plan<L, V>::P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int t = threadIdx.x % TK, q = threadIdx.x / TK;
@@ -37,7 +37,7 @@ k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = active ? base[(long)(q + P * e) * ps] : cx<T>{0, 0};
-    fft_pencil<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, FWD, true, V>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) base[(long)(q + P * e) * ps] = cscale(v[e], scale);
@@ -49,11 +49,11 @@ k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
 // column times exp(i pi (i' + j' + k') / n) (signed indices, shift_field of
 // grid_fft.hh:832-842 as particle_generator.hh:518-522 uses it), transformed
 // from the same loads.  Column t = j * nc + k of an [n0][n1][nc] grid.
-template <typename T, int L, int TK>
-__global__ void __launch_bounds__(Plan<L>::P * TK)
+template <typename T, int L, int TK, int V = 0>
+__global__ void __launch_bounds__((Plan<L, V>::P * TK))
 k_pencils_ax0_shift(cx<T>* __restrict__ data, cx<T>* __restrict__ shifted, int n1, int nc, T scale,
                     const cx<T>* __restrict__ tw, const cx<double>* __restrict__ half, int tkp) {
-    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+    constexpr int E = Plan<L, V>::E, P = Plan<L, V>::P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int t = threadIdx.x % TK, q = threadIdx.x / TK;
@@ -75,12 +75,12 @@ k_pencils_ax0_shift(cx<T>* __restrict__ data, cx<T>* __restrict__ shifted, int n
         const cx<double> ph = half[m];
         w[e] = cmul(v[e], cx<T>{(T)ph.x, (T)ph.y});
     }
-    fft_pencil<T, L, false>(v, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, false, true, V>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) data[col + (long)(q + P * e) * ps] = cscale(v[e], scale);
     }
-    fft_pencil<T, L, false>(w, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, false, true, V>(w, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) shifted[col + (long)(q + P * e) * ps] = cscale(w[e], scale);
@@ -169,10 +169,10 @@ k_rows_c2r(T* __restrict__ data, long nrows, long pitch, T scale, const cx<T>* _
 
 // ------------------------------------------------------------ tables ------
 
-template <int L> static void fill_tw(std::vector<std::complex<long double>>& out) {
+template <int L, int V = 0> static void fill_tw(std::vector<std::complex<long double>>& out) {
     const long double PI = 3.141592653589793238462643383279502884L;
-    for (int s = 0; s < Plan<L>::S; ++s) {
-        const int R = Plan<L>::r(s), NS = plan_ns<L>(s);
+    for (int s = 0; s < Plan<L, V>::S; ++s) {
+        const int R = Plan<L, V>::r(s), NS = plan_ns<L, V>(s);
         for (int k = 0; k < NS; ++k)
             for (int r = 1; r < R; ++r) {
                 const long double a = -2.0L * PI * (long double)((long)r * k % ((long)NS * R)) / (long double)(NS * R);
@@ -195,7 +195,44 @@ bool fft_length_supported(size_t n) {
     }
 }
 
-static std::vector<std::complex<long double>> make_table(int L, int kind) {
+int fft_plan_count(int L) {
+    switch (L) {
+#define X(S) case S: return plan_variants<S>();
+        MFIC_FOR_SIZES(X)
+#undef X
+    default:
+        return 0;
+    }
+}
+
+template <int L, int V> static void plan_info_t(int* elems, int radices[4]) {
+    *elems = Plan<L, V>::E;
+    for (int s = 0; s < 4; ++s) radices[s] = s < Plan<L, V>::S ? Plan<L, V>::r(s) : 0;
+}
+
+void fft_plan_info(int L, int variant, int* elems, int radices[4]) {
+    MFIC_CHECK(variant >= 0 && variant < fft_plan_count(L), MFIC_ERR_ARG,
+               "no radix plan " + std::to_string(variant) + " for length " + std::to_string(L));
+    switch (L) {
+#define X(S)                                                                         \
+    case S:                                                                          \
+        if (variant == 1) plan_info_t<S, alt_plan<S>()>(elems, radices);             \
+        else plan_info_t<S, 0>(elems, radices);                                      \
+        break;
+        MFIC_FOR_SIZES(X)
+#undef X
+    default:
+        throw Error(MFIC_ERR_SIZE, "unsupported FFT length " + std::to_string(L));
+    }
+}
+
+int plan_variant(mfic_ctx ctx, int L, int pass, int prec) {
+    if (ctx->plans.empty()) return 0;
+    const auto it = ctx->plans.find(std::make_tuple(L, pass, prec));
+    return it == ctx->plans.end() ? 0 : it->second;
+}
+
+static std::vector<std::complex<long double>> make_table(int L, int kind, int variant) {
     std::vector<std::complex<long double>> tw;
     if (kind == 2) {  // half-cell shift phases exp(+i pi m / L), m = 0 .. 2L-1
         const long double PI = 3.141592653589793238462643383279502884L;
@@ -206,7 +243,11 @@ static std::vector<std::complex<long double>> make_table(int L, int kind) {
         return tw;
     }
     switch (L) {
-#define X(S) case S: fill_tw<S>(tw); break;
+#define X(S)                                                     \
+    case S:                                                      \
+        if (variant == 1) fill_tw<S, alt_plan<S>()>(tw);         \
+        else fill_tw<S>(tw);                                     \
+        break;
         MFIC_FOR_SIZES(X)
 #undef X
     default:
@@ -225,14 +266,15 @@ static std::vector<std::complex<long double>> make_table(int L, int kind) {
     return tw;
 }
 
-const void* twiddles(mfic_ctx ctx, int L, int kind, int prec) {
+const void* twiddles(mfic_ctx ctx, int L, int kind, int prec, int variant) {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    auto key = std::make_tuple(L, kind, prec);
+    MFIC_CHECK(variant == 0 || (kind == 0 && variant < fft_plan_count(L)), MFIC_ERR_ARG, "no such radix plan");
+    auto key = std::make_tuple(L, kind + 4 * variant, prec);
     auto it = ctx->tables.find(key);
     if (it != ctx->tables.end()) return it->second.dev;
     MFIC_CHECK(!ctx->capturing, MFIC_ERR_STATE,
                "twiddle table for length " + std::to_string(L) + " created during graph capture: run the sequence once before capturing");
-    auto tw = make_table(L, kind);
+    auto tw = make_table(L, kind, variant);
     TwTable t;
     t.count = tw.size();
     size_t bytes = std::max<size_t>(1, t.count) * (prec == MFIC_F64 ? 16 : 8);
@@ -267,10 +309,10 @@ static size_t max_lds(mfic_ctx ctx) {
     return ctx->max_lds;
 }
 
-template <typename T, int L, bool FWD>
-static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tlen, long nouter, T scale,
-                             const char* tag) {
-    constexpr int P = Plan<L>::P;
+template <typename T, int L, bool FWD, int V>
+static void launch_pencils_LV(mfic_ctx ctx, cx<T>* data, long ps, long os, int tlen, long nouter, T scale,
+                              const char* tag) {
+    constexpr int P = Plan<L, V>::P;
     // widest tile (pencils per block) whose LDS fits ~136 KB: 128-byte row
     // segments per pencil position and one resident block per CU measured
     // best on MI355X for the 512 / 768 passes
@@ -281,8 +323,9 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
     if (ov != ctx->tiles.end()) { tk = ov->second.first; pad = ov->second.second; }
     while (tk > 1 && (size_t)L * (tk + pad) * pencil_lds_elem<T, true>() > max_lds(ctx)) tk /= 2;
     while (tk > 1 && P * tk > 1024) tk /= 2;
+    if (V != 0 && tk < 4) tk = 4;  // alternative plans are built for tiles of 4 and more
     const int tkp = tk + pad;
-    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, V));
     const int gx = (tlen + tk - 1) / tk;
     const long ntiles = (long)gx * nouter;
     const bool xcd = nouter > 1;
@@ -291,11 +334,11 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
     ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + (FWD ? "_fwd" : "_inv") + tag + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    2.0 * (double)tlen * L * nouter * sizeof(cx<T>));
     switch (tk) {
-#define TKCASE(K)                                                                                            \
-    case K:                                                                                                  \
-        if constexpr (P * K <= 1024)                                                                         \
-            hipLaunchKernelGGL((k_pencils<T, L, K, FWD>), grid, dim3(P * K), lds, ctx->stream, data, ps, os, \
-                               tlen, scale, tw, tkp, gx, ntiles, xcd ? 1 : 0);                               \
+#define TKCASE(K)                                                                                               \
+    case K:                                                                                                     \
+        if constexpr (P * K <= 1024 && (V == 0 || K >= 4))                                                      \
+            hipLaunchKernelGGL((k_pencils<T, L, K, FWD, V>), grid, dim3(P * K), lds, ctx->stream, data, ps, os, \
+                               tlen, scale, tw, tkp, gx, ntiles, xcd ? 1 : 0);                                  \
         break;
         TKCASE(1) TKCASE(2) TKCASE(4) TKCASE(8) TKCASE(16) TKCASE(32) TKCASE(64)
 #undef TKCASE
@@ -303,6 +346,15 @@ static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tl
         throw Error(MFIC_ERR_SIZE, "bad tile");
     }
     MFIC_LAUNCH_CHECK();
+}
+
+template <typename T, int L, bool FWD>
+static void launch_pencils_L(mfic_ctx ctx, cx<T>* data, long ps, long os, int tlen, long nouter, T scale,
+                             const char* tag) {
+    if (plan_variant(ctx, L, MFIC_PASS_PENCILS, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1)
+        launch_pencils_LV<T, L, FWD, alt_plan<L>()>(ctx, data, ps, os, tlen, nouter, scale, tag);
+    else
+        launch_pencils_LV<T, L, FWD, 0>(ctx, data, ps, os, tlen, nouter, scale, tag);
 }
 
 template <typename T>
@@ -401,9 +453,9 @@ template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], dou
 
 // the two strided c2r passes only (axis 0, axis 1): the contiguous-axis pass
 // is left to a fused row kernel
-template <typename T, int L>
-static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc) {
-    constexpr int P = Plan<L>::P;
+template <typename T, int L, int V>
+static void launch_ax0_shift_v(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc) {
+    constexpr int P = Plan<L, V>::P;
     // at most 16 pencils per tile by default: at 512^3 FP32 the 16-pencil
     // tile takes 0.46 ms per launch, the 32-pencil tile 1.37 ms (two outputs
     // per pencil double the registers of the 1024-thread workgroups)
@@ -414,7 +466,8 @@ static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, 
     const auto ov = ctx->tiles.find(std::make_tuple(L, 2, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) tk = ov->second.first;
     while (tk > 1 && ((size_t)L * tk * pencil_lds_elem<T, true>() > max_lds(ctx) || P * tk > 1024)) tk /= 2;
-    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    if (V != 0 && tk < 4) tk = 4;  // alternative plans are built for tiles of 4 and more
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, V));
     const cx<double>* half = static_cast<const cx<double>*>(twiddles(ctx, L, 2, MFIC_F64));
     const int tlen = n1 * nc;
     dim3 grid((unsigned)((tlen + tk - 1) / tk));
@@ -422,11 +475,11 @@ static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, 
     ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + "_inv_ax0_shift" + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    3.0 * (double)tlen * L * sizeof(cx<T>));
     switch (tk) {
-#define TKCASE(K)                                                                                            \
-    case K:                                                                                                  \
-        if constexpr (P * K <= 1024)                                                                         \
-            hipLaunchKernelGGL((k_pencils_ax0_shift<T, L, K>), grid, dim3(P * K), lds, ctx->stream, data, shifted, \
-                               n1, nc, (T)1, tw, half, K);                                                   \
+#define TKCASE(K)                                                                                               \
+    case K:                                                                                                     \
+        if constexpr (P * K <= 1024 && (V == 0 || K >= 4))                                                      \
+            hipLaunchKernelGGL((k_pencils_ax0_shift<T, L, K, V>), grid, dim3(P * K), lds, ctx->stream, data,    \
+                               shifted, n1, nc, (T)1, tw, half, K);                                             \
         break;
         TKCASE(1) TKCASE(2) TKCASE(4) TKCASE(8) TKCASE(16) TKCASE(32) TKCASE(64)
 #undef TKCASE
@@ -434,6 +487,14 @@ static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, 
         throw Error(MFIC_ERR_SIZE, "bad tile");
     }
     MFIC_LAUNCH_CHECK();
+}
+
+template <typename T, int L>
+static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc) {
+    if (plan_variant(ctx, L, MFIC_PASS_AX0_SHIFT, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1)
+        launch_ax0_shift_v<T, L, alt_plan<L>()>(ctx, data, shifted, n1, nc);
+    else
+        launch_ax0_shift_v<T, L, 0>(ctx, data, shifted, n1, nc);
 }
 
 template <typename T> void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted, const size_t n[3]) {

@@ -9,7 +9,9 @@
 // touch the same addresses (one global read + one global write per element).
 //
 // Radix plans are compile-time per length (all lengths 2^a 3^b up to 768);
-// every radix of a plan divides E.  Inter-stage twiddles come from a table
+// every radix of a plan divides E.  The two padded-pass lengths 512 and 768
+// carry an alternative plan (twice the elements per thread, half the
+// threads per transform) that strided pencil passes can select at run time.  Inter-stage twiddles come from a table
 // computed on the host in long double (rounded once to the working type);
 // intra-butterfly constants are 48th roots of unity.
 #pragma once
@@ -134,13 +136,15 @@ template <typename T, bool FWD> struct Dft<T, 24, FWD> {
 };
 
 // ------------------------------------------------------------ size plans --
-// radices r0 r1 r2 r3 (0 = unused), E = elements per thread
-template <int L> struct Plan;
-#define MFIC_PLAN(L_, E_, S_, A, B, C, D)                                 \
-    template <> struct Plan<L_> {                                          \
-        static constexpr int L = L_, E = E_, S = S_, P = L_ / E_;          \
+// radices r0 r1 r2 r3 (0 = unused), E = elements per thread; V numbers the
+// alternative plans of a length (0 = the default)
+template <int L, int V = 0> struct Plan;
+#define MFIC_PLAN_V(L_, V_, E_, S_, A, B, C, D)                           \
+    template <> struct Plan<L_, V_> {                                      \
+        static constexpr int L = L_, V = V_, E = E_, S = S_, P = L_ / E_;  \
         static constexpr int r(int s) { return s == 0 ? A : s == 1 ? B : s == 2 ? C : D; } \
     };
+#define MFIC_PLAN(L_, E_, S_, A, B, C, D) MFIC_PLAN_V(L_, 0, E_, S_, A, B, C, D)
 MFIC_PLAN(4, 4, 1, 4, 0, 0, 0)
 MFIC_PLAN(6, 6, 1, 6, 0, 0, 0)
 MFIC_PLAN(8, 8, 1, 8, 0, 0, 0)
@@ -164,19 +168,28 @@ MFIC_PLAN(384, 24, 3, 24, 8, 2, 0)
 MFIC_PLAN(512, 16, 3, 16, 8, 4, 0)
 MFIC_PLAN(576, 24, 3, 8, 6, 12, 0)
 MFIC_PLAN(768, 24, 3, 12, 8, 8, 0)
+// alternatives of the padded-pass lengths: 16 threads per transform
+MFIC_PLAN_V(512, 1, 32, 3, 16, 8, 4, 0)
+MFIC_PLAN_V(768, 1, 48, 3, 16, 12, 4, 0)
 #undef MFIC_PLAN
+#undef MFIC_PLAN_V
+// number of radix plans compiled for a length
+template <int L> constexpr int plan_variants() { return (L == 512 || L == 768) ? 2 : 1; }
+// plan 1 of a length that has one, else plan 0 (lets run-time dispatch name
+// plan 1 for every length without instantiating a missing plan)
+template <int L> constexpr int alt_plan() { return plan_variants<L>() > 1 ? 1 : 0; }
 
-template <int L> constexpr int plan_ns(int s) {  // span before stage s
+template <int L, int V = 0> constexpr int plan_ns(int s) {  // span before stage s
     int ns = 1;
-    for (int i = 0; i < s; ++i) ns *= Plan<L>::r(i);
+    for (int i = 0; i < s; ++i) ns *= Plan<L, V>::r(i);
     return ns;
 }
-template <int L> constexpr int plan_tw_offset(int s) {  // twiddle table offset
+template <int L, int V = 0> constexpr int plan_tw_offset(int s) {  // twiddle table offset
     int off = 0;
-    for (int i = 0; i < s; ++i) off += plan_ns<L>(i) * (Plan<L>::r(i) - 1);
+    for (int i = 0; i < s; ++i) off += plan_ns<L, V>(i) * (Plan<L, V>::r(i) - 1);
     return off;
 }
-template <int L> constexpr int plan_tw_size() { return plan_tw_offset<L>(Plan<L>::S); }
+template <int L, int V = 0> constexpr int plan_tw_size() { return plan_tw_offset<L, V>(Plan<L, V>::S); }
 
 // LDS addressing for a thread's transform: address(pos) = base + pos * stride
 struct LdsMap {
@@ -272,12 +285,12 @@ struct WaveSync {
 // SPLIT exchanges the real parts, then the imaginary parts, through an LDS
 // array of T: half the LDS of a complex exchange (more resident workgroups
 // for the wide pencil tiles) for twice the LDS instructions and barriers.
-template <typename T, int L, int S, bool FWD, class MAP, class SYNC = BlockSync, bool SPLIT = false>
+template <typename T, int L, int S, bool FWD, class MAP, class SYNC = BlockSync, bool SPLIT = false, int V = 0>
 __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, const cx<T>* __restrict__ tw) {
-    using PL = Plan<L>;
-    constexpr int E = PL::E, P = PL::P, R = PL::r(S), B = E / R, NS = plan_ns<L>(S);
+    using PL = Plan<L, V>;
+    constexpr int E = PL::E, P = PL::P, R = PL::r(S), B = E / R, NS = plan_ns<L, V>(S);
     constexpr bool LAST = (S == PL::S - 1);
-    const cx<T>* tws = tw + plan_tw_offset<L>(S);
+    const cx<T>* tws = tw + plan_tw_offset<L, V>(S);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = q + P * b;
@@ -335,13 +348,13 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, co
 }
 
 // full transform of length L on registers v[E] (positions q + P e)
-template <typename T, int L, bool FWD, class MAP, class SYNC = BlockSync, bool SPLIT = false>
+template <typename T, int L, bool FWD, class MAP, class SYNC = BlockSync, bool SPLIT = false, int V = 0>
 __device__ __forceinline__ void fft_regs(cx<T>* v, int q, cx<T>* lds, MAP m, const cx<T>* __restrict__ tw) {
-    constexpr int S = Plan<L>::S;
-    fft_stage<T, L, 0, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
-    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
-    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
-    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, MAP, SYNC, SPLIT>(v, q, lds, m, tw);
+    constexpr int S = Plan<L, V>::S;
+    fft_stage<T, L, 0, FWD, MAP, SYNC, SPLIT, V>(v, q, lds, m, tw);
+    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, MAP, SYNC, SPLIT, V>(v, q, lds, m, tw);
+    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, MAP, SYNC, SPLIT, V>(v, q, lds, m, tw);
+    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, MAP, SYNC, SPLIT, V>(v, q, lds, m, tw);
 }
 
 // Pencil passes (TK neighbouring columns per workgroup) with one transform
@@ -353,9 +366,9 @@ template <typename T, bool SPLIT> constexpr size_t pencil_lds_elem() { return SP
 // default LDS budget of a pencil tile: 128-byte row segments per pencil
 // position (16 complex FP32 columns) at 768 points, 32 at 512
 template <bool SPLIT> constexpr size_t pencil_lds_cap() { return SPLIT ? 69632 : 139264; }
-template <typename T, int L, bool FWD, bool SPLIT = true>
+template <typename T, int L, bool FWD, bool SPLIT = true, int V = 0>
 __device__ __forceinline__ void fft_pencil(cx<T>* v, int q, cx<T>* lds, LdsMap m, const cx<T>* __restrict__ tw) {
-    fft_regs<T, L, FWD, LdsMap, BlockSync, SPLIT>(v, q, lds, m, tw);
+    fft_regs<T, L, FWD, LdsMap, BlockSync, SPLIT, V>(v, q, lds, m, tw);
 }
 
 // full transform of a row held by one thread group, exchanges through the

@@ -62,9 +62,10 @@ struct mfic_ctx_s {
     // scratch buffers owned by the context (reused across calls)
     std::map<std::string, std::pair<void*, size_t>> scratch;
     size_t max_lds = 0;  // LDS bytes per workgroup (device attribute)
-    int num_cu = 0;      // compute units of the device
     // tile overrides from the autotuner: (length, kind, prec) -> (tile, lds pad)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tiles;
+    // radix plan choices: (length, pass [mfic_fft_pass], prec) -> plan variant
+    std::map<std::tuple<int, int, int>, int> plans;
     // stream capture into a HIP graph in progress (mfic_graph_begin): no
     // allocation, table creation or event timing may happen meanwhile
     bool capturing = false;
@@ -113,8 +114,16 @@ void* ctx_scratch(mfic_ctx ctx, const std::string& key, size_t bytes);
 
 // twiddles: complex transform of length L (kind 0), or real rows of length
 // 2M (kind 1: complex length M table followed by M+1 untangle factors), or
-// the half-cell shift phases exp(i pi m / L), m < 2L (kind 2, FP64)
-const void* twiddles(mfic_ctx ctx, int L, int kind, int prec);
+// the half-cell shift phases exp(i pi m / L), m < 2L (kind 2, FP64); the
+// inter-stage factors follow radix plan `variant` of the length
+const void* twiddles(mfic_ctx ctx, int L, int kind, int prec, int variant = 0);
+// the radix plan selected for a pass (mfic_fft_pass) of length L (0 unless
+// mfic_fft_set_plan chose another)
+int plan_variant(mfic_ctx ctx, int L, int pass, int prec);
+// number of radix plans compiled for length L
+int fft_plan_count(int L);
+// elements per thread and radices of plan `variant` of length L
+void fft_plan_info(int L, int variant, int* elems, int radices[4]);
 
 // FFT entry points on raw device buffers of layout [n0][n1][n2+2]
 template <typename T> void fft_r2c(mfic_ctx ctx, T* data, const size_t n[3], double scale);

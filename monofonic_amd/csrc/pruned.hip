@@ -38,8 +38,8 @@ namespace mfic {
 // straddle N/2 and N, so band tests and gradient factors cost nothing for the
 // others.
 
-template <int L> struct PadGeo {
-    static constexpr int P = Plan<L>::P, N = 2 * L / 3, NH = N / 2, KC = NH + 1;
+template <int L, int V = 0> struct PadGeo {
+    static constexpr int P = Plan<L, V>::P, N = 2 * L / 3, NH = N / 2, KC = NH + 1;
     static_assert(3 * N == 2 * L, "padded length must be 3N/2");
     // 0: every lane below N/2, 1: every lane above N, 2: every lane in the
     // zero band N/2 < pos <= N, 3: mixed
@@ -125,13 +125,13 @@ template <typename T> struct LdOperand {
         c.ga *= scale;
         return c;
     }
-    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L, V>;
         const int i = G::src(q, e);
         return i < 0 ? cx<T>{0, 0} : c.p[(long)i * (G::N * G::KC)];
     }
-    template <int L> __device__ __forceinline__ cx<T> fix(const Col& c, int q, int e, cx<T> v) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> fix(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L, V>;
         if (G::band(q, e)) return v;
         const double g = G::template grad<double>(q, e, kf);
         const double s = (c.al * g + c.be) * g + c.ga;
@@ -147,13 +147,13 @@ template <typename T> struct LdPow {
     T kf, scale;
     struct Col { const cx<T>* p; };
     template <int L> __device__ __forceinline__ Col column(long col) const { return Col{f + col}; }
-    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L, V>;
         const int i = G::src(q, e);
         return i < 0 ? cx<T>{0, 0} : c.p[(long)i * (G::N * G::KC)];
     }
-    template <int L> __device__ __forceinline__ cx<T> fix(const Col&, int q, int e, cx<T> v) const {
-        const T g = PadGeo<L>::template grad<T>(q, e, kf);
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> fix(const Col&, int q, int e, cx<T> v) const {
+        const T g = PadGeo<L, V>::template grad<T>(q, e, kf);
         const T m = scale * (p > 0 ? g : T(1)) * (p > 1 ? g : T(1));
         return cx<T>{m * v.x, m * v.y};
     }
@@ -172,12 +172,12 @@ template <typename T> struct LdAx1Mul {
         const T gk = grad_t<T>(k, G::N, kf);
         return Col{kb + ip * (L * G::KC) + k, r == 0 ? sign : (r == 1 ? sign * gk : sign * (gk * gk))};
     }
-    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int ln, int e) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> load(const Col& c, int ln, int e) const {
+        using G = PadGeo<L, V>;
         return G::band(ln, e) ? cx<T>{0, 0} : c.p[(long)(ln + G::P * e) * G::KC];
     }
-    template <int L> __device__ __forceinline__ cx<T> fix(const Col& c, int ln, int e, cx<T> v) const {
-        const T g = PadGeo<L>::template grad<T>(ln, e, kf);
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> fix(const Col& c, int ln, int e, cx<T> v) const {
+        const T g = PadGeo<L, V>::template grad<T>(ln, e, kf);
         const T m = c.m * (q > 0 ? g : T(1)) * (q > 1 ? g : T(1));
         return cx<T>{m * v.x, m * v.y};
     }
@@ -192,8 +192,8 @@ template <typename T> struct StPadCol {
         const int j = (int)(col / G::KC), k = (int)(col % G::KC);
         return Col{kb + (long)(j > G::NH ? j + G::NH : j) * G::KC + k};
     }
-    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L, V>;
         c.p[(long)(q + G::P * e) * (L * G::KC)] = v;
     }
 };
@@ -206,12 +206,12 @@ template <typename T, bool SKIP_BAND> struct LdAx1 {
         const long ip = col / G::KC, k = col % G::KC;
         return Col{kb + ip * (L * G::KC) + k};
     }
-    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L, V>;
         if (SKIP_BAND && G::band(q, e)) return cx<T>{0, 0};
         return c.p[(long)(q + G::P * e) * G::KC];
     }
-    template <int L> __device__ __forceinline__ cx<T> fix(const Col&, int, int, cx<T> v) const { return v; }
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> fix(const Col&, int, int, cx<T> v) const { return v; }
 };
 template <typename T, bool DROP_BAND> struct StAx1 {
     cx<T>* kb;
@@ -221,8 +221,8 @@ template <typename T, bool DROP_BAND> struct StAx1 {
         const long ip = col / G::KC, k = col % G::KC;
         return Col{kb + ip * (L * G::KC) + k};
     }
-    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L, V>;
         if (DROP_BAND && G::band(q, e)) return;
         c.p[(long)(q + G::P * e) * G::KC] = v;
     }
@@ -237,11 +237,11 @@ template <typename T> struct LdPadCol {
         const int j = (int)(col / G::KC), k = (int)(col % G::KC);
         return Col{kb + (long)(j > G::NH ? j + G::NH : j) * G::KC + k};
     }
-    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L, V>;
         return c.p[(long)(q + G::P * e) * (L * G::KC)];
     }
-    template <int L> __device__ __forceinline__ cx<T> fix(const Col&, int, int, cx<T> v) const { return v; }
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> fix(const Col&, int, int, cx<T> v) const { return v; }
 };
 // pass A' store: unpad rules + output operator (+ optional -1/k^2) into the N-grid
 template <typename T> struct StUnpad {
@@ -259,8 +259,8 @@ template <typename T> struct StUnpad {
         c.nyq = (j == G::NH) || (k == G::NH);
         return c;
     }
-    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L, V>;
         const int i = G::src(q, e);
         if (i < 0) return;
         double re = 0.0, im = 0.0;
@@ -283,10 +283,10 @@ template <typename T> struct StUnpad {
 };
 // ------------------------------------------------------------- kernels ----
 
-template <typename T, int L, int TK, bool FWD, class LD, class ST>
-__global__ void __launch_bounds__(Plan<L>::P * TK)
+template <typename T, int L, int TK, bool FWD, class LD, class ST, int V = 0>
+__global__ void __launch_bounds__((Plan<L, V>::P * TK))
 k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int xcd) {
-    constexpr int E = Plan<L>::E, P = Plan<L>::P;
+    constexpr int E = Plan<L, V>::E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int t = threadIdx.x % TK, q = threadIdx.x / TK;
@@ -299,14 +299,14 @@ k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int
     // lanes past the last column read column 0 and store nothing, so no load
     // carries a branch
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = ld.template load<L>(lc, q, e);
+    for (int e = 0; e < E; ++e) v[e] = ld.template load<L, V>(lc, q, e);
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = ld.template fix<L>(lc, q, e, v[e]);
-    fft_pencil<T, L, FWD>(v, q, lds, LdsMap{t, tkp}, tw);
+    for (int e = 0; e < E; ++e) v[e] = ld.template fix<L, V>(lc, q, e, v[e]);
+    fft_pencil<T, L, FWD, true, V>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
         const auto sc = st.template column<L>(c);
 #pragma unroll
-        for (int e = 0; e < E; ++e) st.template store<L>(sc, q, e, v[e]);
+        for (int e = 0; e < E; ++e) st.template store<L, V>(sc, q, e, v[e]);
     }
 }
 
@@ -316,12 +316,12 @@ k_pencils_f(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int
 // transform.  Equal to the unpad (convolution.hh:600-651) then pad_insert
 // (convolution.hh:552-594) pair of a two-stage product restricted to that
 // axis; the column mask zeroes kz = N/2 and `scale` carries the pad factors.
-template <typename T, int L, int TK, class LD, class ST>
-__global__ void __launch_bounds__(Plan<L>::P * TK)
+template <typename T, int L, int TK, class LD, class ST, int V = 0>
+__global__ void __launch_bounds__((Plan<L, V>::P * TK))
 k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp, int xcd, int zero_kz_nyquist,
                  T scale) {
-    using G = PadGeo<L>;
-    constexpr int E = Plan<L>::E;
+    using G = PadGeo<L, V>;
+    constexpr int E = Plan<L, V>::E;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int t = threadIdx.x % TK, q = threadIdx.x / TK;
@@ -331,8 +331,8 @@ k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp
     const auto lc = ld.template column<L>(c);
     cx<T> v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = ld.template load<L>(lc, q, e);
-    fft_pencil<T, L, true, false>(v, q, lds, LdsMap{t, tkp}, tw);
+    for (int e = 0; e < E; ++e) v[e] = ld.template load<L, V>(lc, q, e);
+    fft_pencil<T, L, true, false, V>(v, q, lds, LdsMap{t, tkp}, tw);
     const bool colzero = zero_kz_nyquist && (int)(c % G::KC) == G::NH;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -340,11 +340,11 @@ k_pencils_filter(LD ld, ST st, long ncols, const cx<T>* __restrict__ tw, int tkp
         const bool keep = !colzero && !G::band(q, e) && q + G::P * e != G::NH;
         v[e] = keep ? cscale(v[e], scale) : cx<T>{0, 0};
     }
-    fft_pencil<T, L, false, false>(v, q, lds, LdsMap{t, tkp}, tw);
+    fft_pencil<T, L, false, false, V>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
         const auto sc = st.template column<L>(c);
 #pragma unroll
-        for (int e = 0; e < E; ++e) st.template store<L>(sc, q, e, v[e]);
+        for (int e = 0; e < E; ++e) st.template store<L, V>(sc, q, e, v[e]);
     }
 }
 
@@ -354,8 +354,8 @@ template <typename T> struct LdAx0 {
     int np, kc;
     struct Col { const cx<T>* p; };
     template <int L> __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
-    template <int L> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ cx<T> load(const Col& c, int q, int e) const {
+        using G = PadGeo<L, V>;
         return c.p[(long)(q + G::P * e) * (L * G::KC)];
     }
 };
@@ -364,8 +364,8 @@ template <typename T> struct StAx0 {
     int np, kc;
     struct Col { cx<T>* p; };
     template <int L> __device__ __forceinline__ Col column(long col) const { return Col{kb + col}; }
-    template <int L> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
-        using G = PadGeo<L>;
+    template <int L, int V = 0> __device__ __forceinline__ void store(const Col& c, int q, int e, cx<T> v) const {
+        using G = PadGeo<L, V>;
         c.p[(long)(q + G::P * e) * (L * G::KC)] = v;
     }
 };
@@ -484,23 +484,23 @@ k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long 
 
 // ------------------------------------------------------------- launchers --
 
-template <typename T, int L, bool SPLIT>
+template <typename T, int L, bool SPLIT, int V>
 static int pick_tile(mfic_ctx ctx) {
-    constexpr int P = Plan<L>::P;
+    constexpr int P = Plan<L, V>::P;
     int tk = 64;
     size_t cap = pencil_lds_cap<SPLIT>();
     const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) { tk = ov->second.first; cap = 139264; }
     while (tk > 1 && ((size_t)L * tk * pencil_lds_elem<T, SPLIT>() > cap || P * tk > 1024)) tk /= 2;
-    return tk;
+    return V != 0 && tk < 4 ? 4 : tk;  // alternative plans are built for tiles of 4 and more
 }
 
-template <typename T, int L, bool FWD, class LD, class ST>
-static void launch_f(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
-                     bool xcd = false) {
-    constexpr int P = Plan<L>::P;
-    const int tk = pick_tile<T, L, true>(ctx);
-    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+template <typename T, int L, bool FWD, class LD, class ST, int V>
+static void launch_fv(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
+                      bool xcd) {
+    constexpr int P = Plan<L, V>::P;
+    const int tk = pick_tile<T, L, true, V>(ctx);
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, V));
     const size_t lds = (size_t)L * tk * pencil_lds_elem<T, true>();
     // axis-1 passes: 16-column tiles of rows with an odd pitch share cache
     // lines with their neighbours, which the XCD-aware order keeps in one L2
@@ -508,11 +508,11 @@ static void launch_f(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const
     dim3 grid((unsigned)(xcd ? xcd_grid(ntiles) : ntiles));
     ProfScope prof(ctx, std::string(name) + "_L" + std::to_string(L) + (sizeof(T) == 8 ? "_f64" : "_f32"), bytes);
     switch (tk) {
-#define TKC(K)                                                                                                     \
-    case K:                                                                                                        \
-        if constexpr (P * K <= 1024)                                                                               \
-            hipLaunchKernelGGL((k_pencils_f<T, L, K, FWD, LD, ST>), grid, dim3(P * K), lds, ctx->stream, ld, st, ncols, \
-                               tw, K, xcd ? 1 : 0);                                                                \
+#define TKC(K)                                                                                                  \
+    case K:                                                                                                     \
+        if constexpr (P * K <= 1024 && (V == 0 || K >= 4))                                                      \
+            hipLaunchKernelGGL((k_pencils_f<T, L, K, FWD, LD, ST, V>), grid, dim3(P * K), lds, ctx->stream, ld, \
+                               st, ncols, tw, K, xcd ? 1 : 0);                                                  \
         break;
         TKC(1) TKC(2) TKC(4) TKC(8) TKC(16) TKC(32) TKC(64)
 #undef TKC
@@ -520,6 +520,16 @@ static void launch_f(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const
         throw Error(MFIC_ERR_SIZE, "bad tile");
     }
     MFIC_LAUNCH_CHECK();
+}
+
+// a strided padded pass with the radix plan selected for its kind (pass)
+template <typename T, int L, bool FWD, class LD, class ST>
+static void launch_f(mfic_ctx ctx, int pass, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
+                     bool xcd = false) {
+    if (plan_variant(ctx, L, pass, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1)
+        launch_fv<T, L, FWD, LD, ST, alt_plan<L>()>(ctx, ld, st, ncols, name, bytes, xcd);
+    else
+        launch_fv<T, L, FWD, LD, ST, 0>(ctx, ld, st, ncols, name, bytes, xcd);
 }
 
 template <typename T, int M>
@@ -601,8 +611,8 @@ static void c2r_from_operand_t(mfic_ctx ctx, const OperandDesc& o, size_t n, dou
     switch (np) {
 #define X(S)                                                                                                 \
     case S:                                                                                                  \
-        launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs);                    \
-        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs, true);                    \
+        launch_f<T, S, false>(ctx, MFIC_PASS_PAD_C2R_A, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs);                    \
+        launch_f<T, S, false>(ctx, MFIC_PASS_PAD_C2R_B, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs, true);                    \
         rows_c2r_M<T, S / 2>(ctx, static_cast<const cx<T>*>(kbuf), kc, static_cast<T*>(rout), np, (long)np * np, \
                              (T)norm);                                                                       \
         break;
@@ -648,8 +658,8 @@ static void r2c_products_t(mfic_ctx ctx, const std::vector<ProdTerm>& terms, siz
 #define X(S)                                                                                                        \
     case S:                                                                                                         \
         rows_r2c_M<T, S / 2>(ctx, pr, np, static_cast<cx<T>*>(kbuf), kc, (long)np * np, (T)1);                      \
-        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs, true);                            \
-        launch_f<T, S, true>(ctx, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
+        launch_f<T, S, true>(ctx, MFIC_PASS_PAD_R2C_B, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs, true);                            \
+        launch_f<T, S, true>(ctx, MFIC_PASS_PAD_R2C_A, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
         break;
         MFIC_PADDED_SIZES(X)
 #undef X
@@ -672,8 +682,8 @@ static void c2r_to_kbuf_t(mfic_ctx ctx, const OperandDesc& o, size_t n, double L
     switch (np) {
 #define X(S)                                                                                   \
     case S:                                                                                    \
-        launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs); \
-        launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs, true); \
+        launch_f<T, S, false>(ctx, MFIC_PASS_PAD_C2R_A, la, sa, colsA, "fft_c2rA", (double)colsA * (N + np) * cs); \
+        launch_f<T, S, false>(ctx, MFIC_PASS_PAD_C2R_B, lb, sb, colsB, "fft_c2rB", (double)colsB * (N + np) * cs, true); \
         break;
         MFIC_PADDED_SIZES(X)
 #undef X
@@ -697,8 +707,8 @@ static void r2c_from_kbuf_t(mfic_ctx ctx, size_t n, double L, void* kbuf, void* 
     switch (np) {
 #define X(S)                                                                                                        \
     case S:                                                                                                         \
-        launch_f<T, S, true>(ctx, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs, true);                            \
-        launch_f<T, S, true>(ctx, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
+        launch_f<T, S, true>(ctx, MFIC_PASS_PAD_R2C_B, lb, sb, colsB, "fft_r2cB", (double)colsB * (np + N) * cs, true);                            \
+        launch_f<T, S, true>(ctx, MFIC_PASS_PAD_R2C_A, la, sa, colsA, "fft_r2cA", (double)colsA * (np + N * (op == MFIC_OP_ASSIGN ? 1 : 2)) * cs); \
         break;
         MFIC_PADDED_SIZES(X)
 #undef X
@@ -728,12 +738,12 @@ static void hessians_to_kbuf_t(mfic_ctx ctx, const void* f, size_t n, double L, 
         for (int p = 2; p >= 0; --p) {                                                                         \
             LdPow<T> la{static_cast<const cx<T>*>(f), N, kc, p, (T)kf, (T)rfac};                               \
             StPadCol<T> sa{xb + p * stride, N, np, kc};                                                        \
-            launch_f<T, S, false>(ctx, la, sa, colsA, "fft_c2rAh", (double)colsA * (N + np) * cs);            \
+            launch_f<T, S, false>(ctx, MFIC_PASS_PAD_C2R_A, la, sa, colsA, "fft_c2rAh", (double)colsA * (N + np) * cs);            \
             for (int h = 0; h < 6; ++h) {                                                                      \
                 if (PQR[h][0] != p) continue;                                                                  \
                 LdAx1Mul<T> lb{xb + p * stride, N, np, kc, PQR[h][1], PQR[h][2], (T)kf, (T)-1};                \
                 StAx1<T, false> sb{ob + h * ostride};                                                   \
-                launch_f<T, S, false>(ctx, lb, sb, colsB, "fft_c2rBh", (double)colsB * (N + np) * cs, true);        \
+                launch_f<T, S, false>(ctx, MFIC_PASS_PAD_C2R_B, lb, sb, colsB, "fft_c2rBh", (double)colsB * (N + np) * cs, true);        \
             }                                                                                                  \
         }                                                                                                      \
         break;
@@ -750,12 +760,12 @@ void padded_hessians_to_kbuf(mfic_ctx ctx, int prec, const void* f, size_t n, do
     else hessians_to_kbuf_t<float>(ctx, f, n, L, xbuf, out, stride);
 }
 
-template <typename T, int L, class LD, class ST>
-static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, bool xcd, int zk, double scale,
-                          const char* name) {
-    constexpr int P = Plan<L>::P;
-    const int tk = pick_tile<T, L, false>(ctx);
-    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+template <typename T, int L, class LD, class ST, int V>
+static void launch_filter_v(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, bool xcd, int zk, double scale,
+                            const char* name) {
+    constexpr int P = Plan<L, V>::P;
+    const int tk = pick_tile<T, L, false, V>(ctx);
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, V));
     const size_t lds = (size_t)L * tk * pencil_lds_elem<T, false>();
     const long ntiles = (ncols + tk - 1) / tk;
     dim3 grid((unsigned)(xcd ? xcd_grid(ntiles) : ntiles));
@@ -764,9 +774,9 @@ static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, 
     switch (tk) {
 #define TKC(K)                                                                                                  \
     case K:                                                                                                     \
-        if constexpr (P * K <= 1024)                                                                            \
-            hipLaunchKernelGGL((k_pencils_filter<T, L, K, LD, ST>), grid, dim3(P * K), lds, ctx->stream, ld, st, \
-                               ncols, tw, K, xcd ? 1 : 0, zk, (T)scale);                                        \
+        if constexpr (P * K <= 1024 && (V == 0 || K >= 4))                                                      \
+            hipLaunchKernelGGL((k_pencils_filter<T, L, K, LD, ST, V>), grid, dim3(P * K), lds, ctx->stream, ld, \
+                               st, ncols, tw, K, xcd ? 1 : 0, zk, (T)scale);                                    \
         break;
         TKC(1) TKC(2) TKC(4) TKC(8) TKC(16) TKC(32) TKC(64)
 #undef TKC
@@ -774,6 +784,15 @@ static void launch_filter(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, 
         throw Error(MFIC_ERR_SIZE, "bad tile");
     }
     MFIC_LAUNCH_CHECK();
+}
+
+template <typename T, int L, class LD, class ST>
+static void launch_filter(mfic_ctx ctx, int pass, const LD& ld, const ST& st, long ncols, bool xcd, int zk,
+                          double scale, const char* name) {
+    if (plan_variant(ctx, L, pass, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1)
+        launch_filter_v<T, L, LD, ST, alt_plan<L>()>(ctx, ld, st, ncols, xcd, zk, scale, name);
+    else
+        launch_filter_v<T, L, LD, ST, 0>(ctx, ld, st, ncols, xcd, zk, scale, name);
 }
 
 template <typename T>
@@ -789,8 +808,8 @@ static void truncate_repad_t(mfic_ctx ctx, size_t n, void* kbuf) {
     switch (np) {
 #define X(S)                                                                                       \
     case S:                                                                                        \
-        launch_filter<T, S>(ctx, l1, s1, cols, true, 1, norm, "fft_truncate_ax1");       \
-        launch_filter<T, S>(ctx, l0, s0, cols, false, 0, 1.0, "fft_truncate_ax0");       \
+        launch_filter<T, S>(ctx, MFIC_PASS_TRUNC_AX1, l1, s1, cols, true, 1, norm, "fft_truncate_ax1");       \
+        launch_filter<T, S>(ctx, MFIC_PASS_TRUNC_AX0, l0, s0, cols, false, 0, 1.0, "fft_truncate_ax0");       \
         break;
         MFIC_PADDED_SIZES(X)
 #undef X

@@ -111,3 +111,27 @@ def test_context_closes_children_before_itself(monkeypatch):
     assert calls[-1] == "mfic_ctx_destroy"
     assert sorted(calls[:-1]) == ["child0", "child1", "child2"]
     assert not ctx.h
+
+
+def test_radix_plans():
+    """every compiled radix plan multiplies to its length, each radix divides
+    the elements per thread, and a transform fits one wavefront group; the
+    padded-pass lengths carry the alternative plan"""
+    from monofonic_amd import batch
+    sizes = [4, 6, 8, 9, 12, 16, 18, 24, 32, 36, 48, 64, 72, 96, 128, 144, 192, 256, 288, 384, 512, 576, 768]
+    for length in sizes:
+        count = batch.plan_count(length)
+        assert count == (2 if length in (512, 768) else 1), length
+        for v in range(count):
+            e, radices = batch.plan_info(length, v)
+            assert int(np.prod(radices)) == length and all(e % r == 0 for r in radices), (length, v)
+            assert length % e == 0 and length // e <= 64
+    assert batch.plan_info(768, 1) == (48, (16, 12, 4))
+    assert batch.plan_info(512, 1) == (32, (16, 8, 4))
+    assert batch.plan_count(7) == 0 and batch.plan_count(1024) == 0
+    with pytest.raises(L.MficError):
+        batch.plan_info(384, 1)
+    # a context is needed to select a plan: without one the call fails cleanly
+    assert L.load().mfic_fft_set_plan(None, 768, L.PASS_PAD_C2R_B, L.F32, 1) == -1
+    v = C.c_int()
+    assert L.load().mfic_fft_get_plan(None, 768, L.PASS_PAD_C2R_B, L.F32, C.byref(v)) == -1

@@ -120,15 +120,32 @@ def test_c2_potentials_and_orders_match_oracle(ctx, prec):
 
 
 # -------------------------------------------------------------------- C3 --
+def _set_plans(ctx, n, prec, variant):
+    """every strided pass of the n^3 run (N-grid pencils, shifted
+    sub-lattice pass, padded convolver passes) on radix plan `variant`
+    (-1: back to the defaults)"""
+    from monofonic_amd import batch
+    for length in (n, 3 * n // 2):
+        if batch.plan_count(length) > 1:
+            for p in L.PASSES:
+                batch.set_plan(ctx, length, p, prec, variant)
+
+
 @pytest.mark.gpu
+@pytest.mark.parametrize("plans", ["default", "alternative"])
 @pytest.mark.parametrize("prec", [F32, F64])
-def test_c3_particles_match_oracle(ctx, prec):
+def test_c3_particles_match_oracle(ctx, prec, plans):
     s = load_sample("c3_sample.npz")
     n, box, z, seed = int(s["n"]), float(s["box"]), float(s["zstart"]), int(s["seed"])
-    _flush(f"C3 {n}^3 3LPT+PLT bcc particles prec={prec}")
+    _flush(f"C3 {n}^3 3LPT+PLT bcc particles prec={prec} radix plans {plans}")
     run = lpt.ICRun(ctx, lpt.ICConfig(n=n, box=box, zstart=z, lpt_order=3, lattice=L.BCC, plt=True, prec=prec,
                                       seed=seed, particles=True))
-    run.execute()
+    if plans == "alternative":
+        _set_plans(ctx, n, prec, 1)
+    try:
+        run.execute()
+    finally:
+        _set_plans(ctx, n, prec, -1)
     pos_d, vel_d, npart = run.particles_device()
     assert npart == 2 * n ** 3
     parts = s["particles"]

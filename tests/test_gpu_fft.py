@@ -41,6 +41,27 @@ def test_fft_matches_numpy(ctx, shape, prec, tol):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(768, 16, 16), (16, 768, 16), (512, 24, 16), (24, 512, 16)])
+@pytest.mark.parametrize("prec,tol", [(F64, 1e-12), (F32, 2e-6)])
+def test_fft_alternative_radix_plan(ctx, shape, prec, tol):
+    """the strided passes on the alternative radix plan of their length"""
+    from monofonic_amd import _lib as L
+    from monofonic_amd import batch
+    length = max(shape[:2])
+    assert batch.plan_count(length) == 2
+    print(f"fft shape={shape} prec={prec} plan 1 {batch.plan_info(length, 1)}", flush=True)
+    batch.set_plan(ctx, length, L.PASS_PENCILS, prec, 1)
+    try:
+        assert batch.get_plan(ctx, length, L.PASS_PENCILS, prec) == 1
+        k, ref, back, x = _roundtrip(ctx, shape, prec, seed=7)
+    finally:
+        batch.set_plan(ctx, length, L.PASS_PENCILS, prec, -1)
+    assert batch.get_plan(ctx, length, L.PASS_PENCILS, prec) == 0
+    assert np.linalg.norm(k - ref) / np.linalg.norm(ref) < tol
+    assert np.linalg.norm(back - x) / np.linalg.norm(x) < tol
+
+
+@pytest.mark.gpu
 def test_fft_single_mode(ctx):
     """a single cosine mode maps to two half-amplitude delta peaks"""
     n = 64
