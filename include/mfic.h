@@ -116,12 +116,13 @@ int mfic_grid_shift(mfic_grid g, const double s[3], int transform_back);
 int mfic_fft_forward(mfic_grid g);
 int mfic_fft_backward(mfic_grid g);
 int mfic_fft_size_supported(size_t n);
-/* autotuner hook: override the tile (pencils per block for kind 0 strided
- * passes and for kind 2, the bcc particle path's axis-0 pass that also forms
- * the shifted sub-lattice; rows per block for kind 1 contiguous-axis
- * passes) and the LDS pitch
- * padding for one transform length and precision; tile <= 0 restores the
- * built-in choice.  No reference counterpart (FFTW planning,
+/* autotuner hook: override the tile (pencils per block for kind 0, every
+ * strided pass of the length, and for kind 2, the bcc particle path's axis-0
+ * pass that also forms the shifted sub-lattice; rows per block for kind 1
+ * contiguous-axis passes; kinds 3..8, the padded convolver pass kinds of
+ * mfic_fft_pass below, override kind 0 for that pass only) and the LDS pitch
+ * padding (kinds 0 and 1) for one transform length and precision; tile <= 0
+ * restores the built-in choice.  No reference counterpart (FFTW planning,
  * grid_fft.cc:53-54, plays this role there). */
 int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, int lds_pad);
 
@@ -345,7 +346,8 @@ int mfic_graph_destroy(mfic_graph g);
 /* Autotuner: times whole pipeline executions of a temporary run while it
  * walks, one knob at a time, the radix plan of every strided pass kind that
  * has more than one (mfic_fft_set_plan), the pencils-per-block choices of the
- * N-grid and padded strided passes and of the shifted sub-lattice pass, and
+ * N-grid strided passes, of the padded passes (all of a length, then each
+ * padded pass kind) and of the shifted sub-lattice pass, and
  * the rows-per-block choices of the N-grid contiguous pass
  * (mfic_fft_set_tile), then the plans once more; the winners stay in the
  * context.  mfic_autotune tunes an n^3 box of precision prec and LPT order

@@ -359,8 +359,9 @@ int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, in
     return guard([&] {
         MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
         MFIC_CHECK(fft_length_supported(length), MFIC_ERR_SIZE, "unsupported FFT length");
-        MFIC_CHECK(kind >= 0 && kind <= 2, MFIC_ERR_ARG,
-                   "kind must be 0 (strided pencils), 1 (rows) or 2 (axis-0 pass with the shifted sub-lattice)");
+        MFIC_CHECK(kind >= 0 && kind <= MFIC_PASS_TRUNC_AX1, MFIC_ERR_ARG,
+                   "kind must be 0 (strided pencils), 1 (rows), 2 (axis-0 pass with the shifted sub-lattice) or a "
+                   "padded pass kind 3..8");
         MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
         const auto key = std::make_tuple(length, kind, prec);
         if (tile <= 0) { ctx->tiles.erase(key); return; }

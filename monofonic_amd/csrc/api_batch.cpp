@@ -76,8 +76,9 @@ namespace {
 
 // Coordinate descent over the radix plan of every strided pass kind and the
 // tile of every transform length a run configuration uses (pencils per
-// block for the strided N-grid and padded passes and for the axis-0 pass of
-// the shifted sub-lattice, rows per block for the contiguous N-grid pass),
+// block for the strided N-grid and padded passes, then for each padded pass
+// kind and the axis-0 pass of the shifted sub-lattice, rows per block for
+// the contiguous N-grid pass),
 // timing whole executions of a temporary run of that configuration with HIP
 // events: plans, then tiles, then the plans again under the chosen tiles.
 // Leaves the winners in the context and returns the best time.
@@ -127,7 +128,12 @@ double autotune_config(mfic_ctx ctx, const mfic_ic_config& cfg, const mfic_cosmo
         struct Knob { int length, kind; std::vector<int> cand; };
         std::vector<Knob> tile_knobs = {{n, 0, {4, 8, 16, 32, 64}}, {n / 2, 1, {1, 2, 4, 8, 16}}};
         if (shift) tile_knobs.push_back({n, 2, {4, 8, 16, 32}});
-        if (cfg.lpt_order > 1) tile_knobs.push_back({np, 0, {4, 8, 16, 32, 64}});
+        if (cfg.lpt_order > 1) {
+            // one tile for every padded pass, then each pass kind on its own
+            tile_knobs.push_back({np, 0, {4, 8, 16, 32, 64}});
+            for (int pass = MFIC_PASS_PAD_C2R_A; pass <= MFIC_PASS_TRUNC_AX1; ++pass)
+                tile_knobs.push_back({np, pass, {4, 8, 16, 32}});
+        }
 
         ck(mfic_run_execute(run, 1));
         float cur = time_once();

@@ -484,22 +484,27 @@ k_rows_r2c_prod(RowProd pr, long rpitch, cx<T>* __restrict__ kout, int kc, long 
 
 // ------------------------------------------------------------- launchers --
 
+// tile of a padded pass: the override of its own kind, else the override of
+// every strided pass of the length (kind 0), else the widest tile whose LDS
+// fits the default budget
 template <typename T, int L, bool SPLIT, int V>
-static int pick_tile(mfic_ctx ctx) {
+static int pick_tile(mfic_ctx ctx, int pass) {
     constexpr int P = Plan<L, V>::P;
     int tk = 64;
     size_t cap = pencil_lds_cap<SPLIT>();
-    const auto ov = ctx->tiles.find(std::make_tuple(L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    const int prec = sizeof(T) == 8 ? MFIC_F64 : MFIC_F32;
+    auto ov = ctx->tiles.find(std::make_tuple(L, pass, prec));
+    if (ov == ctx->tiles.end()) ov = ctx->tiles.find(std::make_tuple(L, 0, prec));
     if (ov != ctx->tiles.end()) { tk = ov->second.first; cap = 139264; }
     while (tk > 1 && ((size_t)L * tk * pencil_lds_elem<T, SPLIT>() > cap || P * tk > 1024)) tk /= 2;
     return V != 0 && tk < 4 ? 4 : tk;  // alternative plans are built for tiles of 4 and more
 }
 
 template <typename T, int L, bool FWD, class LD, class ST, int V>
-static void launch_fv(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
+static void launch_fv(mfic_ctx ctx, int pass, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
                       bool xcd) {
     constexpr int P = Plan<L, V>::P;
-    const int tk = pick_tile<T, L, true, V>(ctx);
+    const int tk = pick_tile<T, L, true, V>(ctx, pass);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, V));
     const size_t lds = (size_t)L * tk * pencil_lds_elem<T, true>();
     // axis-1 passes: 16-column tiles of rows with an odd pitch share cache
@@ -527,9 +532,9 @@ template <typename T, int L, bool FWD, class LD, class ST>
 static void launch_f(mfic_ctx ctx, int pass, const LD& ld, const ST& st, long ncols, const char* name, double bytes,
                      bool xcd = false) {
     if (plan_variant(ctx, L, pass, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1)
-        launch_fv<T, L, FWD, LD, ST, alt_plan<L>()>(ctx, ld, st, ncols, name, bytes, xcd);
+        launch_fv<T, L, FWD, LD, ST, alt_plan<L>()>(ctx, pass, ld, st, ncols, name, bytes, xcd);
     else
-        launch_fv<T, L, FWD, LD, ST, 0>(ctx, ld, st, ncols, name, bytes, xcd);
+        launch_fv<T, L, FWD, LD, ST, 0>(ctx, pass, ld, st, ncols, name, bytes, xcd);
 }
 
 template <typename T, int M>
@@ -761,10 +766,10 @@ void padded_hessians_to_kbuf(mfic_ctx ctx, int prec, const void* f, size_t n, do
 }
 
 template <typename T, int L, class LD, class ST, int V>
-static void launch_filter_v(mfic_ctx ctx, const LD& ld, const ST& st, long ncols, bool xcd, int zk, double scale,
-                            const char* name) {
+static void launch_filter_v(mfic_ctx ctx, int pass, const LD& ld, const ST& st, long ncols, bool xcd, int zk,
+                            double scale, const char* name) {
     constexpr int P = Plan<L, V>::P;
-    const int tk = pick_tile<T, L, false, V>(ctx);
+    const int tk = pick_tile<T, L, false, V>(ctx, pass);
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, V));
     const size_t lds = (size_t)L * tk * pencil_lds_elem<T, false>();
     const long ntiles = (ncols + tk - 1) / tk;
@@ -790,9 +795,9 @@ template <typename T, int L, class LD, class ST>
 static void launch_filter(mfic_ctx ctx, int pass, const LD& ld, const ST& st, long ncols, bool xcd, int zk,
                           double scale, const char* name) {
     if (plan_variant(ctx, L, pass, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1)
-        launch_filter_v<T, L, LD, ST, alt_plan<L>()>(ctx, ld, st, ncols, xcd, zk, scale, name);
+        launch_filter_v<T, L, LD, ST, alt_plan<L>()>(ctx, pass, ld, st, ncols, xcd, zk, scale, name);
     else
-        launch_filter_v<T, L, LD, ST, 0>(ctx, ld, st, ncols, xcd, zk, scale, name);
+        launch_filter_v<T, L, LD, ST, 0>(ctx, pass, ld, st, ncols, xcd, zk, scale, name);
 }
 
 template <typename T>
