@@ -20,7 +20,8 @@ at 256^3: timing, FP64 and finite-difference gradient checks).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
        python bench.py --calibrate-copy          (copy kernel for counter scaling)
-       python bench.py --summarize-pmc DIR       (counter table from rocprofv3 runs)
+       python bench.py --summarize-pmc DIR       (traffic table from rocprofv3 runs)
+       python bench.py --summarize-counters MATCH DIR...  (counter means of one kernel)
 """
 import argparse
 import ctypes as C
@@ -461,11 +462,33 @@ def summarize_pmc(d):
         print(f"{k}\t{c}\t{f * 1024 * ff:.0f}\t{w * 1024 * fw:.0f}")
 
 
+def summarize_counters(dirs, match):
+    """mean per dispatch of every counter of rocprofv3 --pmc runs (one
+    directory per pass) for the kernels whose name contains `match`"""
+    import collections
+    acc, cnt = collections.defaultdict(float), collections.Counter()
+    names = set()
+    for d in dirs:
+        for r in _pmc_rows(d):
+            if match in r["Kernel_Name"]:
+                names.add(r["Kernel_Name"])
+                acc[r["Counter_Name"]] += float(r["Counter_Value"])
+                cnt[r["Counter_Name"]] += 1
+    for k in sorted(names):
+        print(f"# kernel: {k}")
+    print("# counter\tmean per dispatch\tdispatches")
+    for c in sorted(acc):
+        print(f"{c}\t{acc[c] / cnt[c]:.6g}\t{cnt[c]}")
+
+
 def main():
     if "--calibrate-copy" in sys.argv:
         return calibrate_copy()
     if "--summarize-pmc" in sys.argv:
         return summarize_pmc(sys.argv[sys.argv.index("--summarize-pmc") + 1])
+    if "--summarize-counters" in sys.argv:
+        i = sys.argv.index("--summarize-counters")
+        return summarize_counters(sys.argv[i + 2:], sys.argv[i + 1])
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -477,6 +500,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--plans", default="",
+                    help="fixed radix plans instead of autotuning: LENGTH:PASS:VARIANT[,...] (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); every box of this workload fits one "
@@ -490,6 +515,12 @@ def main():
     ctx = Context(0)
     cfg = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=BCC, plt=True, prec=F32, seed=12345,
                    particles=True)
+    if args.plans:
+        from monofonic_amd.batch import set_plan
+        for item in args.plans.split(","):
+            length, pass_kind, variant = (int(x) for x in item.split(":"))
+            set_plan(ctx, length, pass_kind, F32, variant)
+        args.no_tune = True
     if not args.no_tune:
         from monofonic_amd.batch import autotune_config, chosen_plans
         from monofonic_amd.lpt import baseline_cosmology
