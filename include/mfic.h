@@ -433,6 +433,25 @@ int mfic_gadget_write(mfic_run r, const char *fname, const mfic_gadget_opts *opt
 int mfic_gadget_write_host(const char *fname, const mfic_ic_config *cfg, const mfic_cosmo_params *cosmo,
                            const mfic_gadget_opts *opts, const double *pos, const double *vel);
 
+/* The SWIFT output plugin (output_swift.cc:36-377, registered for FP64):
+ * /Units, /ICs_parameters, /PartType1 with Coordinates and Velocities
+ * [N][3] FP64, ParticleIDs (uint64 with UseLongids, the plugin's default,
+ * else uint32) and Masses [N] FP64, every dataset chunked with a Fletcher-32
+ * checksum (HDF_IO.hh:888-963; chunks are cut to the dataset length), and
+ * /Header as the plugin's destructor writes it. */
+typedef struct {
+    int long_ids; /* [output] UseLongids (the plugin defaults it to 1) */
+} mfic_swift_opts;
+/* position_unit / velocity_unit / mass_unit of the SWIFT plugin
+ * (output_swift.cc:63-69): BoxLength / h [Mpc], BoxLength [km/s],
+ * 27.7536609198 BoxLength^3 / h [1e10 Msun]; a run written with
+ * mfic_swift_write must use the first two as lunit and vunit */
+int mfic_swift_units(double boxlength, const mfic_cosmo_params *cosmo, double *lunit, double *vunit,
+                     double *munit);
+int mfic_swift_write(mfic_run r, const char *fname, const mfic_swift_opts *opts);
+int mfic_swift_write_host(const char *fname, const mfic_ic_config *cfg, const mfic_cosmo_params *cosmo,
+                          const mfic_swift_opts *opts, const double *pos, const double *vel);
+
 #ifdef __cplusplus
 }
 #endif

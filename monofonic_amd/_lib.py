@@ -77,6 +77,10 @@ class GadgetOpts(C.Structure):
     _fields_ = [("double_precision", C.c_int), ("long_ids", C.c_int), ("gadget2_compat", C.c_int)]
 
 
+class SwiftOpts(C.Structure):
+    _fields_ = [("long_ids", C.c_int)]
+
+
 SIGNATURES = {
     "mfic_last_error": (C.c_char_p, []),
     "mfic_version": (C.c_char_p, []),
@@ -135,6 +139,11 @@ SIGNATURES = {
     "mfic_gadget_write": (C.c_int, [_P, C.c_char_p, C.POINTER(GadgetOpts)]),
     "mfic_gadget_write_host": (C.c_int, [C.c_char_p, C.POINTER(IcConfig), C.POINTER(CosmoParams),
                                          C.POINTER(GadgetOpts), _P, _P]),
+    "mfic_swift_units": (C.c_int, [C.c_double, C.POINTER(CosmoParams), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "mfic_swift_write": (C.c_int, [_P, C.c_char_p, C.POINTER(SwiftOpts)]),
+    "mfic_swift_write_host": (C.c_int, [C.c_char_p, C.POINTER(IcConfig), C.POINTER(CosmoParams),
+                                        C.POINTER(SwiftOpts), _P, _P]),
     "mfic_run_destroy": (C.c_int, [_P]),
     "mfic_run_execute": (C.c_int, [_P, C.c_int]),
     "mfic_run_field": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),

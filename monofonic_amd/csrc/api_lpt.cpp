@@ -489,6 +489,57 @@ int mfic_gadget_write_host(const char* fname, const mfic_ic_config* cfg, const m
     });
 }
 
+int mfic_swift_units(double boxlength, const mfic_cosmo_params* cosmo, double* lunit, double* vunit, double* munit) {
+    return guard_call([&] {
+        MFIC_CHECK(cosmo && lunit && vunit && munit, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(boxlength > 0 && cosmo->H0 > 0, MFIC_ERR_CONFIG, "bad box length or H0");
+        swift_units(boxlength, *cosmo, lunit, vunit, munit);
+    });
+}
+
+static void check_swift_units(const mfic_ic_config& c, const mfic_cosmo_params& cosmo) {
+    double lu, vu, mu;
+    swift_units(c.L, cosmo, &lu, &vu, &mu);
+    MFIC_CHECK(std::fabs(c.lunit - lu) <= 1e-12 * lu && std::fabs(c.vunit - vu) <= 1e-12 * vu, MFIC_ERR_CONFIG,
+               "particles must be in the SWIFT units: lunit = " + std::to_string(lu) + ", vunit = " +
+                   std::to_string(vu) + " (mfic_swift_units)");
+}
+
+int mfic_swift_write(mfic_run r, const char* fname, const mfic_swift_opts* opts) {
+    return guard_call([&] {
+        MFIC_CHECK(r && fname && opts, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(r->cfg.particles && r->d_pos && r->d_vel, MFIC_ERR_STATE, "run was configured for field output");
+        check_swift_units(r->cfg, r->cosmo);
+        ParticleSource src;
+        src.ctx = r->ctx;
+        src.pos = r->d_pos;
+        src.vel = r->d_vel;
+        src.npart = r->npart;
+        src.ncell = r->cfg.n * r->cfg.n * r->cfg.n;
+        src.overload = 1u << r->cfg.lattice;
+        MFIC_HIP(hipSetDevice(r->ctx->device));
+        swift_write(fname, r->cfg, r->cosmo, *opts, src, r->d_tk != nullptr);
+    });
+}
+
+int mfic_swift_write_host(const char* fname, const mfic_ic_config* cfg, const mfic_cosmo_params* cosmo,
+                          const mfic_swift_opts* opts, const double* pos, const double* vel) {
+    return guard_call([&] {
+        MFIC_CHECK(fname && cfg && cosmo && opts && pos && vel, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(cfg->lattice == MFIC_LATTICE_SC || cfg->lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
+                   "ParticleLoad must be sc or bcc");
+        MFIC_CHECK(cfg->n > 0, MFIC_ERR_SIZE, "empty lattice");
+        check_swift_units(*cfg, *cosmo);
+        ParticleSource src;
+        src.pos = pos;
+        src.vel = vel;
+        src.ncell = cfg->n * cfg->n * cfg->n;
+        src.overload = 1u << cfg->lattice;
+        src.npart = src.ncell * src.overload;
+        swift_write(fname, *cfg, *cosmo, *opts, src, cosmo->tk_n > 0);
+    });
+}
+
 int mfic_fft_c2c(mfic_ctx ctx, void* data, const size_t n[3], int prec, int forward) {
     return guard_call([&] {
         MFIC_CHECK(ctx && data && n, MFIC_ERR_ARG, "null argument");

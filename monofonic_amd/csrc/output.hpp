@@ -1,6 +1,7 @@
 // Particle output writers (output.hip).
 #pragma once
 #include <cstddef>
+#include <cstdint>
 
 #include "mfic_internal.hpp"
 
@@ -14,6 +15,7 @@ struct ParticleSource {
     const double *pos = nullptr, *vel = nullptr;
     size_t npart = 0, ncell = 0;
     unsigned overload = 1;
+    uint64_t id_offset = 0;  // IDoffset * overload * ncell (particle_generator.hh:231)
 };
 
 // output_plugin units of the Gadget-HDF5 writer (output_gadget_hdf5.cc:79-85)
@@ -22,5 +24,10 @@ void gadget_units(double boxlength, double zstart, double* lunit, double* vunit,
 // tabulated transfer function (recorded in ICs_parameters)
 void gadget_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_params& cosmo,
                   const mfic_gadget_opts& opts, const ParticleSource& src, bool tabulated);
+// output_plugin units of the SWIFT writer (output_swift.cc:63-69)
+void swift_units(double boxlength, const mfic_cosmo_params& cosmo, double* lunit, double* vunit, double* munit);
+// one SWIFT initial-condition file of a run's particles
+void swift_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_params& cosmo,
+                 const mfic_swift_opts& opts, const ParticleSource& src, bool tabulated);
 
 }  // namespace mfic

@@ -41,6 +41,11 @@ def lib():
             ("H5Aopen_by_name", i64, [i64, C.c_char_p, C.c_char_p, i64, i64]), ("H5Aclose", C.c_int, [i64]),
             ("H5Aget_type", i64, [i64]), ("H5Aget_space", i64, [i64]),
             ("H5Aread", C.c_int, [i64, i64, C.c_void_p]), ("H5Lexists", C.c_int, [i64, C.c_char_p, i64]),
+            ("H5Dget_create_plist", i64, [i64]), ("H5Pclose", C.c_int, [i64]),
+            ("H5Pget_layout", C.c_int, [i64]), ("H5Pget_chunk", C.c_int, [i64, C.c_int, C.POINTER(C.c_ulonglong)]),
+            ("H5Pget_nfilters", C.c_int, [i64]),
+            ("H5Pget_filter2", C.c_int, [i64, C.c_uint, C.POINTER(C.c_uint), C.POINTER(C.c_size_t),
+                                         C.POINTER(C.c_uint), C.c_size_t, C.c_char_p, C.POINTER(C.c_uint)]),
         ):
             f = getattr(h, fn)
             f.restype, f.argtypes = res, args
@@ -94,6 +99,27 @@ class File:
         assert h.H5Dread(d, t, 0, 0, 0, out.ctypes.data) >= 0
         h.H5Tclose(t), h.H5Sclose(sp), h.H5Dclose(d)
         return out, dt
+
+    def storage(self, path):
+        """(chunk shape or None when contiguous, filter ids) of a dataset"""
+        h = self.h
+        d = h.H5Dopen2(self.f, path.encode(), 0)
+        if d < 0:
+            raise KeyError(path)
+        pl = h.H5Dget_create_plist(d)
+        chunk = None
+        if h.H5Pget_layout(pl) == 2:  # H5D_CHUNKED
+            dims = (C.c_ulonglong * 4)()
+            nd = h.H5Pget_chunk(pl, 4, dims)
+            chunk = tuple(int(dims[i]) for i in range(nd))
+        filters = []
+        for i in range(h.H5Pget_nfilters(pl)):
+            flags, nel, cd = C.c_uint(), C.c_size_t(0), (C.c_uint * 1)()
+            name = C.create_string_buffer(64)
+            conf = C.c_uint()
+            filters.append(h.H5Pget_filter2(pl, i, C.byref(flags), C.byref(nel), cd, 64, name, C.byref(conf)))
+        h.H5Pclose(pl), h.H5Dclose(d)
+        return chunk, filters
 
     def attr(self, group, name):
         """(value, numpy dtype, shape) of a group attribute; scalars have

@@ -115,3 +115,74 @@ def test_writer_rejects_other_units(tmp_path):
         write_gadget_hdf5_host(str(tmp_path / "x.hdf5"), cfg, lpt.baseline_cosmology(24.0), pos, pos)
     assert e.value.code == -6
     assert not os.path.exists(tmp_path / "x.hdf5")
+
+
+# ------------------------------------------------------------------ SWIFT --
+def test_swift_units():
+    from monofonic_amd.output import swift_units
+    cos = lpt.baseline_cosmology(24.0)
+    lu, vu, mu = swift_units(100.0, cos)
+    assert lu == 100.0 / 0.7 and vu == 100.0 and mu == 27.7536609198 * 100.0 ** 3 / 0.7
+
+
+@pytest.mark.parametrize("lattice,long_ids", [(0, True), (1, False)])
+def test_swift_file_from_host_arrays(tmp_path, lattice, long_ids):
+    """the layout of output_swift.cc:36-377 with HDF_IO.hh: Units, FP64
+    vectors, per-particle Masses, chunked datasets with Fletcher-32 (filter
+    id 3), the 7-entry header"""
+    from monofonic_amd.output import swift_config, swift_units, write_swift_host
+    n, box, z = 8, 100.0, 24.0
+    cos = lpt.baseline_cosmology(z)
+    cfg = swift_config(ICConfig(n=n, box=box, zstart=z, lpt_order=2, lattice=lattice, seed=7), cos)
+    npart = (1 << lattice) * n ** 3
+    rng = np.random.default_rng(5 + lattice)
+    pos = rng.uniform(0, box / 0.7, (npart, 3))
+    vel = rng.normal(0, 300.0, (npart, 3))
+    fname = str(tmp_path / "ics_swift.hdf5")
+    write_swift_host(fname, cfg, cos, pos, vel, long_ids=long_ids)
+    _, _, munit = swift_units(box, cos)
+    f = hdf.File(fname)
+    try:
+        c, dt = f.dataset("PartType1/Coordinates")
+        assert dt == np.float64 and np.array_equal(c, pos)
+        v, dt = f.dataset("PartType1/Velocities")
+        assert dt == np.float64 and np.array_equal(v, vel)
+        ids, dt = f.dataset("PartType1/ParticleIDs")
+        assert dt == (np.uint64 if long_ids else np.uint32)
+        assert np.array_equal(ids.astype(np.uint64), reference_ids(n, lattice))
+        m, dt = f.dataset("PartType1/Masses")
+        assert dt == np.float64 and m.shape == (npart,) and np.all(m == 0.3 * munit / npart)
+        # chunks: 256 KB of vector rows, 1 MB of scalars, cut to the length
+        assert f.storage("PartType1/Coordinates") == ((min(npart, 32768), 3), [3])
+        assert f.storage("PartType1/Masses") == ((min(npart, 131072),), [3])
+        assert f.storage("PartType1/ParticleIDs") == ((min(npart, (1 << 20) // (8 if long_ids else 4)),), [3])
+        for name, want in (("Unit mass in cgs (U_M)", 1.98841e43), ("Unit length in cgs (U_L)", 3.08567758149e24),
+                           ("Unit time in cgs (U_t)", 3.08567758149e19), ("Unit current in cgs (U_I)", 1.0),
+                           ("Unit temperature in cgs (U_T)", 1.0)):
+            val, dt, shape = f.attr("Units", name)
+            assert dt == np.float64 and shape == () and val == want, name
+        assert f.attr("Header", "Dimension")[:3] == (3, np.dtype(np.int32), ())
+        assert f.attr("Header", "BoxSize")[0] == box / 0.7
+        tot, dt, shape = f.attr("Header", "NumPart_Total")
+        assert dt == np.uint32 and shape == (7,) and list(tot) == [0, npart, 0, 0, 0, 0, 0]
+        assert list(f.attr("Header", "NumPart_Total_HighWord")[0]) == [0] * 7
+        this, dt, _ = f.attr("Header", "NumPart_ThisFile")
+        assert dt == np.uint64 and list(this) == [0, npart, 0, 0, 0, 0, 0]
+        mass, dt, shape = f.attr("Header", "MassTable")
+        assert dt == np.float64 and shape == (7,) and not mass.any()
+        for name, want, dtype in (("Time", 1.0 / (1.0 + z), np.float64), ("Redshift", z, np.float64),
+                                  ("Flag_Entropy_ICs", 0, np.int32), ("NumFilesPerSnapshot", 1, np.int32)):
+            val, dt, shape = f.attr("Header", name)
+            assert dt == dtype and shape == (1,) and val[0] == want, name
+        assert f.attr("ICs_parameters", "Particle Load")[0] == (b"bcc" if lattice else b"sc")
+    finally:
+        f.close()
+
+
+def test_swift_writer_rejects_gadget_units(tmp_path):
+    from monofonic_amd.output import write_swift_host
+    cfg = gadget_config(ICConfig(n=4, box=50.0, zstart=24.0))
+    pos = np.zeros((64, 3))
+    with pytest.raises(L.MficError) as e:
+        write_swift_host(str(tmp_path / "y.hdf5"), cfg, lpt.baseline_cosmology(24.0), pos, pos)
+    assert e.value.code == -6
