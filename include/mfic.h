@@ -288,12 +288,19 @@ int mfic_lpt_component(mfic_ctx ctx, int order, int comp, const mfic_lpt_coef *k
 /* lattice_generator::set_positions / set_velocities for Bravais sc / bcc,
  * lattice index 0 (particle_generator.hh:499-544, 597-639): `field` is the
  * real-space component; positions are ((ijk + shift)/N) * lunit + field in
- * FP64.  dst is a device or host pointer to 3*npart doubles (interleaved
- * [ipart][3]); is_pos = 0 writes velocities (lunit ignored).  For bcc the
+ * FP64.  dst is device memory for 3*npart doubles (interleaved [ipart][3];
+ * host memory is refused with MFIC_ERR_ARG); is_pos = 0 writes velocities
+ * (lunit ignored).  For bcc the
  * second sub-lattice is read after a Fourier shift by (1/2,1/2,1/2) cells
  * (the field is left shifted, as the reference leaves it). */
 int mfic_lattice_readout(mfic_grid field, int lattice, int idim, int is_pos, double lunit,
                          double *dst_device);
+/* the same for lattice index 0 or 1: index 1 (the secondary lattice of the
+ * baryon species) first shifts the field by second_lattice_shift (sc
+ * (1/2,1/2,1/2), bcc (1/2,1/2,0); particle_generator.hh:58-63, 512-531)
+ * and offsets the positions by it */
+int mfic_lattice_readout_index(mfic_grid field, int lattice, int lattice_index, int idim, int is_pos,
+                               double lunit, double *dst);
 
 /* --------------------------------------------------------------- pipeline --- */
 typedef struct {
@@ -307,6 +314,8 @@ typedef struct {
     long seed;              /* [random] seed                                 */
     int particles;          /* 0: 6 component fields, 1: FP64 particles     */
     double lunit, vunit;    /* output plugin position_unit / velocity_unit   */
+    int baryons;            /* DoBaryons: a second, baryon species on the
+                             * secondary lattice (particle output, E&H)     */
 } mfic_ic_config;
 
 /* The whole hot path of ic_generator::run for one box (ic_generator.cc:
@@ -322,6 +331,15 @@ int mfic_run_execute(mfic_run r, int blocking);
 int mfic_run_field(mfic_run r, int comp, mfic_grid *out);
 /* particle arrays (device pointers, 3*npart doubles each) when particles=1 */
 int mfic_run_particles(mfic_run r, double **pos, double **vel, size_t *npart);
+/* the baryon species of a run with baryons = 1 (ic_generator.cc:704-745):
+ * the same LPT displacements and velocities (delta_bc = theta_bc = 0 with
+ * the Eisenstein & Hu fit, so C_species has no effect) read out on the
+ * secondary lattice, offset by second_lattice_shift (particle_generator.hh:
+ * 58-63: sc (1/2,1/2,1/2), bcc (1/2,1/2,0)); sub-lattice s at that offset plus
+ * s (1/2,1/2,1/2), positions not wrapped.  Device pointers, [p][3] FP64,
+ * npart as for the dark matter.  The dark matter takes Omega_m - Omega_b,
+ * the baryons Omega_b (ic_generator.cc:157-166). */
+int mfic_run_baryon_particles(mfic_run r, double **pos, double **vel, size_t *npart);
 /* k-space potentials phi1, phi2, phi3, A3x, A3y, A3z (which = 0..5), before
  * growth scaling: the assembly applies g1..g3c (ic_generator.cc:644-658) */
 int mfic_run_potential(mfic_run r, int which, mfic_grid *out);
@@ -426,7 +444,10 @@ typedef struct {
 int mfic_gadget_units(double boxlength, double zstart, double *lunit, double *vunit, double *munit);
 /* write_particle_data + the header of a run with particles = 1 (sc or bcc),
  * streamed chunk by chunk from its device arrays (the run must have
- * executed); cosmological header values come from the run's parameters */
+ * executed); cosmological header values come from the run's parameters.  A
+ * run with baryons adds /PartType0 (IDs after the dark matter's) and, as the
+ * plugin does for individually massive particles, a Masses dataset per
+ * species (32-bit floats, 64-bit with double_precision) and a zero MassTable. */
 int mfic_gadget_write(mfic_run r, const char *fname, const mfic_gadget_opts *opts);
 /* the same file from host arrays: FP64 [p][3] positions and velocities of
  * the 2^lattice * n^3 particles of cfg (cfg->lunit / vunit as above) */
@@ -438,9 +459,14 @@ int mfic_gadget_write_host(const char *fname, const mfic_ic_config *cfg, const m
  * [N][3] FP64, ParticleIDs (uint64 with UseLongids, the plugin's default,
  * else uint32) and Masses [N] FP64, every dataset chunked with a Fletcher-32
  * checksum (HDF_IO.hh:888-963; chunks are cut to the dataset length), and
- * /Header as the plugin's destructor writes it. */
+ * /Header as the plugin's destructor writes it.  A run with baryons adds
+ * /PartType0 with Masses, InternalEnergy (the gas temperature of
+ * output_swift.cc:86-104 from Tcmb, Omega_b, h, YHe and gamma) and
+ * SmoothingLength (BoxLength / h / GridRes). */
 typedef struct {
-    int long_ids; /* [output] UseLongids (the plugin defaults it to 1) */
+    int long_ids; /* [output] UseLongids (the plugin defaults it to 1)            */
+    double YHe;   /* [cosmology] YHe for the gas internal energy (<= 0: 0.245421) */
+    double gamma; /* [cosmology] gamma, adiabatic index (<= 0: 5/3)              */
 } mfic_swift_opts;
 /* position_unit / velocity_unit / mass_unit of the SWIFT plugin
  * (output_swift.cc:63-69): BoxLength / h [Mpc], BoxLength [km/s],

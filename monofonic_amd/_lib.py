@@ -65,7 +65,8 @@ class Operand(C.Structure):
 class IcConfig(C.Structure):
     _fields_ = [("n", C.c_size_t), ("L", C.c_double), ("lpt_order", C.c_int), ("lattice", C.c_int),
                 ("plt", C.c_int), ("prec", C.c_int), ("do_fixing", C.c_int), ("do_inversion", C.c_int),
-                ("seed", C.c_long), ("particles", C.c_int), ("lunit", C.c_double), ("vunit", C.c_double)]
+                ("seed", C.c_long), ("particles", C.c_int), ("lunit", C.c_double), ("vunit", C.c_double),
+                ("baryons", C.c_int)]
 
 
 _P = C.c_void_p
@@ -78,7 +79,7 @@ class GadgetOpts(C.Structure):
 
 
 class SwiftOpts(C.Structure):
-    _fields_ = [("long_ids", C.c_int)]
+    _fields_ = [("long_ids", C.c_int), ("YHe", C.c_double), ("gamma", C.c_double)]
 
 
 SIGNATURES = {
@@ -131,6 +132,7 @@ SIGNATURES = {
     "mfic_plt_download": (C.c_int, [_P, _P, _P, _P, _P]),
     "mfic_lpt_component": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(LptCoef), _P, C.c_double] + [_P] * 7),
     "mfic_lattice_readout": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, _P]),
+    "mfic_lattice_readout_index": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _P]),
     "mfic_run_create": (C.c_int, [_P, C.POINTER(IcConfig), C.POINTER(CosmoParams), C.POINTER(_P)]),
     "mfic_fft_c2c": (C.c_int, [_P, _P, C.POINTER(C.c_size_t), C.c_int, C.c_int]),
     "mfic_ppt_fields": (C.c_int, [_P, C.c_double, _P, _P, _P, _P, C.POINTER(C.c_double)]),
@@ -148,6 +150,7 @@ SIGNATURES = {
     "mfic_run_execute": (C.c_int, [_P, C.c_int]),
     "mfic_run_field": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "mfic_run_particles": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "mfic_run_baryon_particles": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_size_t)]),
     "mfic_run_potential": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "mfic_graph_begin": (C.c_int, [_P]),
     "mfic_graph_end": (C.c_int, [_P, C.POINTER(_P)]),

@@ -78,6 +78,21 @@ static void check_grid(mfic_grid g) {
 
 using namespace mfic;
 
+namespace mfic {
+
+void require_device_memory(const void* p, const char* what) {
+    hipPointerAttribute_t at{};
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // unknown to the runtime: not device memory
+        at.type = hipMemoryTypeUnregistered;
+    }
+    MFIC_CHECK(at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged, MFIC_ERR_ARG,
+               std::string(what) + " must be device memory (host memory is refused)");
+}
+
+}  // namespace mfic
+
 extern "C" {
 
 const char* mfic_last_error(void) { return g_last_error.c_str(); }

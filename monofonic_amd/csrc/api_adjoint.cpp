@@ -82,6 +82,8 @@ int mfic_grid_wrap(mfic_ctx ctx, const size_t n[3], const double L[3], int prec,
         MFIC_CHECK(n[0] > 0 && n[1] > 0 && n[2] > 0 && n[2] % 2 == 0, MFIC_ERR_SIZE, "bad grid size");
         MFIC_CHECK(((uintptr_t)device_ptr % (prec == MFIC_F64 ? 16 : 8)) == 0, MFIC_ERR_ARG,
                    "device buffer must be aligned to one complex element");
+        MFIC_HIP(hipSetDevice(ctx->device));
+        mfic::require_device_memory(device_ptr, "the wrapped buffer");
         auto* g = new mfic_grid_s();
         g->ctx = ctx;
         for (int i = 0; i < 3; ++i) { g->n[i] = n[i]; g->L[i] = L[i]; }

@@ -44,6 +44,7 @@ struct mfic_run_s {
     uint32_t* d_table = nullptr;
     void* d_tk = nullptr;  // spline nodes of a tabulated transfer function
     double *d_pos = nullptr, *d_vel = nullptr;
+    double *d_pos_b = nullptr, *d_vel_b = nullptr;  // baryon species (secondary lattice)
     size_t npart = 0;
     mfic::PltTables plt;
     bool has_plt = false;
@@ -116,6 +117,8 @@ mfic_run_s::~mfic_run_s() {
     if (d_tk) (void)hipFree(d_tk);
     if (d_pos) (void)hipFree(d_pos);
     if (d_vel) (void)hipFree(d_vel);
+    if (d_pos_b) (void)hipFree(d_pos_b);
+    if (d_vel_b) (void)hipFree(d_vel_b);
 }
 
 extern "C" {
@@ -275,23 +278,39 @@ int mfic_lpt_component(mfic_ctx ctx, int order, int comp, const mfic_lpt_coef* k
     });
 }
 
-int mfic_lattice_readout(mfic_grid field, int lattice, int idim, int is_pos, double lunit, double* dst) {
+int mfic_lattice_readout_index(mfic_grid field, int lattice, int lattice_index, int idim, int is_pos, double lunit,
+                               double* dst) {
     return guard_call([&] {
         need(field, "field");
         MFIC_CHECK(dst, MFIC_ERR_ARG, "null destination");
         MFIC_CHECK(lattice == MFIC_LATTICE_SC || lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
                    "readout supports sc and bcc lattices");
+        MFIC_CHECK(lattice_index == 0 || lattice_index == 1, MFIC_ERR_ARG, "lattice index must be 0 or 1");
+        MFIC_HIP(hipSetDevice(field->ctx->device));
+        require_device_memory(dst, "the readout destination");
         MFIC_CHECK(idim >= 0 && idim < 3, MFIC_ERR_ARG, "idim must be 0..2");
         MFIC_CHECK(field->space == MFIC_RSPACE, MFIC_ERR_STATE, "field must be in real space");
         const int n = (int)field->n[0];
-        lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, 0.0, 0, dst);
+        double g[3] = {0.0, 0.0, 0.0};
+        if (lattice_index == 1) {
+            // the secondary lattice: a global shift first (particle_generator.hh:58-63, 512-516)
+            second_lattice_shift(lattice, g);
+            int rc = mfic_grid_shift(field, g, 1);
+            if (rc != MFIC_OK) throw Error(rc, mfic_last_error());
+        }
+        lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, g[idim], 0, dst);
         if (lattice == MFIC_LATTICE_BCC) {
             const double s[3] = {0.5, 0.5, 0.5};
             int rc = mfic_grid_shift(field, s, 1);
             if (rc != MFIC_OK) throw Error(rc, mfic_last_error());
-            lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, 0.5, (size_t)n * n * n, dst);
+            lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, 0.5 + g[idim],
+                                (size_t)n * n * n, dst);
         }
     });
+}
+
+int mfic_lattice_readout(mfic_grid field, int lattice, int idim, int is_pos, double lunit, double* dst) {
+    return mfic_lattice_readout_index(field, lattice, 0, idim, is_pos, lunit, dst);
 }
 
 int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_params* cosmo, mfic_run* out) {
@@ -304,6 +323,10 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
         MFIC_CHECK(cfg->n % 2 == 0 || cfg->lpt_order == 1, MFIC_ERR_SIZE,
                    "Orszag convolver for LPTorder>1 requires even grid size!");
         MFIC_CHECK(cfg->L > 0, MFIC_ERR_CONFIG, "BoxLength must be positive");
+        MFIC_CHECK(!cfg->baryons || cfg->particles, MFIC_ERR_UNSUPPORTED,
+                   "baryon species are built for particle output only");
+        MFIC_CHECK(!cfg->baryons || cosmo->tk_n == 0, MFIC_ERR_UNSUPPORTED,
+                   "baryon species need the Eisenstein & Hu transfer function (delta_bc = 0)");
         auto r = std::make_unique<mfic_run_s>();
         r->ctx = ctx;
         r->cfg = *cfg;
@@ -323,10 +346,15 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
         for (int p = 0; p < (cfg->lpt_order == 1 ? 1 : cfg->lpt_order == 2 ? 2 : 6); ++p)
             r->pot[p] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
         if (cfg->particles) {
-            for (int q = 0; q < 6 * (1 << cfg->lattice); ++q) r->comp[q] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
+            const int ncomp = (cfg->lattice == MFIC_LATTICE_BCC || cfg->baryons) ? 12 : 6;
+            for (int q = 0; q < ncomp; ++q) r->comp[q] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
             r->npart = (size_t(1) << cfg->lattice) * n * n * n;
             r->d_pos = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "particle positions"));
             r->d_vel = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "particle velocities"));
+            if (cfg->baryons) {
+                r->d_pos_b = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "baryon positions"));
+                r->d_vel_b = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "baryon velocities"));
+            }
         } else {
             for (int c = 0; c < 6; ++c) r->fields[c] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_RSPACE);
         }
@@ -398,8 +426,37 @@ int mfic_run_execute(mfic_run r, int blocking) {
                 const void* fp[3] = {og[6 * s]->d, og[6 * s + 1]->d, og[6 * s + 2]->d};
                 const void* fv[3] = {og[6 * s + 3]->d, og[6 * s + 4]->d, og[6 * s + 5]->d};
                 const size_t pbase = (size_t)s * c.n * c.n * c.n;
-                rows_c2r_particles(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, 0.5 * s, pbase, r->d_pos);
-                rows_c2r_particles(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, 0.5 * s, pbase, r->d_vel);
+                const double sh[3] = {0.5 * s, 0.5 * s, 0.5 * s};
+                rows_c2r_particles(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, sh, pbase, r->d_pos);
+                rows_c2r_particles(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, sh, pbase, r->d_vel);
+            }
+            if (c.baryons) {
+                // the baryon species on the secondary lattice (ic_generator.cc:
+                // 716-745, particle_generator.hh:512-531): with delta_bc = 0
+                // (Eisenstein & Hu) its displacements and velocities are the
+                // same fields read out at the lattice shifted by
+                // second_lattice_shift, sub-lattice s at second + s/2 cells.
+                // The components are assembled again (the dark matter pass
+                // transformed them in place) and each axis-0 pass forms both
+                // sub-lattices' phases from one load.
+                assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3,
+                             outs, 1);
+                double g[3];
+                second_lattice_shift(c.lattice, g);
+                const int wa[3] = {(int)(2 * g[0]), (int)(2 * g[1]), (int)(2 * g[2])};
+                const int wb[3] = {wa[0] + 1, wa[1] + 1, wa[2] + 1};
+                for (int q = 0; q < 6; ++q) {
+                    if (c.prec == MFIC_F64) fft_c2r_cols_shift<double>(r->ctx, (double*)og[q]->d, (double*)og[q + 6]->d, nn, wa, wb);
+                    else fft_c2r_cols_shift<float>(r->ctx, (float*)og[q]->d, (float*)og[q + 6]->d, nn, wa, wb);
+                }
+                for (int s = 0; s < nsub; ++s) {
+                    const void* fp[3] = {og[6 * s]->d, og[6 * s + 1]->d, og[6 * s + 2]->d};
+                    const void* fv[3] = {og[6 * s + 3]->d, og[6 * s + 4]->d, og[6 * s + 5]->d};
+                    const size_t pbase = (size_t)s * c.n * c.n * c.n;
+                    const double sh[3] = {g[0] + 0.5 * s, g[1] + 0.5 * s, g[2] + 0.5 * s};
+                    rows_c2r_particles(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, sh, pbase, r->d_pos_b);
+                    rows_c2r_particles(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, sh, pbase, r->d_vel_b);
+                }
             }
         } else {
             for (int q = 0; q < 6; ++q) {
@@ -430,12 +487,33 @@ int mfic_run_particles(mfic_run r, double** pos, double** vel, size_t* npart) {
     });
 }
 
+int mfic_run_baryon_particles(mfic_run r, double** pos, double** vel, size_t* npart) {
+    return guard_call([&] {
+        MFIC_CHECK(r, MFIC_ERR_ARG, "null run");
+        MFIC_CHECK(r->cfg.particles && r->cfg.baryons, MFIC_ERR_STATE, "run has no baryon species");
+        if (pos) *pos = r->d_pos_b;
+        if (vel) *vel = r->d_vel_b;
+        if (npart) *npart = r->npart;
+    });
+}
+
 int mfic_run_potential(mfic_run r, int which, mfic_grid* out) {
     return guard_call([&] {
         MFIC_CHECK(r && out && which >= 0 && which < 6, MFIC_ERR_ARG, "bad argument");
         MFIC_CHECK(r->pot[which] != nullptr, MFIC_ERR_STATE, "potential not computed for this LPT order");
         *out = r->pot[which];
     });
+}
+
+// the baryon species of a run as a particle source: the secondary lattice's
+// particles with IDs after the dark matter's (IDoffset 1,
+// ic_generator.cc:722-724, particle_generator.hh:231)
+static ParticleSource baryon_source(mfic_run r, const ParticleSource& dm) {
+    ParticleSource b = dm;
+    b.pos = r->d_pos_b;
+    b.vel = r->d_vel_b;
+    b.id_offset = (uint64_t)dm.overload * dm.ncell;
+    return b;
 }
 
 int mfic_gadget_units(double boxlength, double zstart, double* lunit, double* vunit, double* munit) {
@@ -466,8 +544,9 @@ int mfic_gadget_write(mfic_run r, const char* fname, const mfic_gadget_opts* opt
         src.npart = r->npart;
         src.ncell = r->cfg.n * r->cfg.n * r->cfg.n;
         src.overload = 1u << r->cfg.lattice;
+        ParticleSource bsrc = baryon_source(r, src);
         MFIC_HIP(hipSetDevice(r->ctx->device));
-        gadget_write(fname, r->cfg, r->cosmo, *opts, src, r->d_tk != nullptr);
+        gadget_write(fname, r->cfg, r->cosmo, *opts, src, r->cfg.baryons ? &bsrc : nullptr, r->d_tk != nullptr);
     });
 }
 
@@ -485,7 +564,8 @@ int mfic_gadget_write_host(const char* fname, const mfic_ic_config* cfg, const m
         src.ncell = cfg->n * cfg->n * cfg->n;
         src.overload = 1u << cfg->lattice;
         src.npart = src.ncell * src.overload;
-        gadget_write(fname, *cfg, *cosmo, *opts, src, cosmo->tk_n > 0);
+        MFIC_CHECK(!cfg->baryons, MFIC_ERR_UNSUPPORTED, "host arrays carry one species: write baryon runs from the run");
+        gadget_write(fname, *cfg, *cosmo, *opts, src, nullptr, cosmo->tk_n > 0);
     });
 }
 
@@ -517,8 +597,9 @@ int mfic_swift_write(mfic_run r, const char* fname, const mfic_swift_opts* opts)
         src.npart = r->npart;
         src.ncell = r->cfg.n * r->cfg.n * r->cfg.n;
         src.overload = 1u << r->cfg.lattice;
+        ParticleSource bsrc = baryon_source(r, src);
         MFIC_HIP(hipSetDevice(r->ctx->device));
-        swift_write(fname, r->cfg, r->cosmo, *opts, src, r->d_tk != nullptr);
+        swift_write(fname, r->cfg, r->cosmo, *opts, src, r->cfg.baryons ? &bsrc : nullptr, r->d_tk != nullptr);
     });
 }
 
@@ -536,7 +617,8 @@ int mfic_swift_write_host(const char* fname, const mfic_ic_config* cfg, const mf
         src.ncell = cfg->n * cfg->n * cfg->n;
         src.overload = 1u << cfg->lattice;
         src.npart = src.ncell * src.overload;
-        swift_write(fname, *cfg, *cosmo, *opts, src, cosmo->tk_n > 0);
+        MFIC_CHECK(!cfg->baryons, MFIC_ERR_UNSUPPORTED, "host arrays carry one species: write baryon runs from the run");
+        swift_write(fname, *cfg, *cosmo, *opts, src, nullptr, cosmo->tk_n > 0);
     });
 }
 
@@ -544,6 +626,8 @@ int mfic_fft_c2c(mfic_ctx ctx, void* data, const size_t n[3], int prec, int forw
     return guard_call([&] {
         MFIC_CHECK(ctx && data && n, MFIC_ERR_ARG, "null argument");
         MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
+        MFIC_HIP(hipSetDevice(ctx->device));
+        require_device_memory(data, "the complex grid");
         const double norm = 1.0 / std::sqrt((double)n[0] * (double)n[1] * (double)n[2]);
         if (prec == MFIC_F64) fft_c2c<double>(ctx, data, n, forward != 0, norm);
         else fft_c2c<float>(ctx, data, n, forward != 0, norm);

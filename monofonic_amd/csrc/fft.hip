@@ -49,10 +49,13 @@ k_pencils(cx<T>* __restrict__ data, long ps, long os, int tlen, T scale,
 // column times exp(i pi (i' + j' + k') / n) (signed indices, shift_field of
 // grid_fft.hh:832-842 as particle_generator.hh:518-522 uses it), transformed
 // from the same loads.  Column t = j * nc + k of an [n0][n1][nc] grid.
-template <typename T, int L, int TK, int V = 0>
+// GEN: both outputs carry a shift of wa / 2 and wb / 2 cells per axis
+// (weights 0, 1 or 2: the secondary lattices of particle_generator.hh:58-63
+// and their sub-lattices), the in-place output included.
+template <typename T, int L, int TK, int V = 0, bool GEN = false>
 __global__ void __launch_bounds__((Plan<L, V>::P * TK))
 k_pencils_ax0_shift(cx<T>* __restrict__ data, cx<T>* __restrict__ shifted, int n1, int nc, T scale,
-                    const cx<T>* __restrict__ tw, const cx<double>* __restrict__ half, int tkp) {
+                    const cx<T>* __restrict__ tw, const cx<double>* __restrict__ half, int tkp, int3 wa, int3 wb) {
     constexpr int E = Plan<L, V>::E, P = Plan<L, V>::P;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
@@ -63,17 +66,31 @@ k_pencils_ax0_shift(cx<T>* __restrict__ data, cx<T>* __restrict__ shifted, int n
     const int col = active ? tg : 0;
     const long ps = (long)tlen;
     const int j = col / nc, k = col - (col / nc) * nc;
-    const int mc = (j - (j > n1 / 2) * n1) + k;  // kz index k <= n/2 is its own signed index
+    const int js = j - (j > n1 / 2) * n1;  // kz index k <= n/2 is its own signed index
+    const int mc = GEN ? wb.y * js + wb.z * k : js + k;
+    const int ma = GEN ? wa.y * js + wa.z * k : 0;
     cx<T> v[E], w[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = data[col + (long)(q + P * e) * ps];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int i = q + P * e;
-        int m = (i - (i > L / 2) * L) + mc;
-        m = m < 0 ? m + 2 * L : (m >= 2 * L ? m - 2 * L : m);
-        const cx<double> ph = half[m];
-        w[e] = cmul(v[e], cx<T>{(T)ph.x, (T)ph.y});
+        const int i = q + P * e, is = i - (i > L / 2) * L;
+        if (GEN) {
+            // exp(i pi m / L) with m = w . (signed i, j, k), reduced mod 2L
+            int m = (wb.x * is + mc) % (2 * L);
+            m += m < 0 ? 2 * L : 0;
+            const cx<double> ph = half[m];
+            w[e] = cmul(v[e], cx<T>{(T)ph.x, (T)ph.y});
+            int m0 = (wa.x * is + ma) % (2 * L);
+            m0 += m0 < 0 ? 2 * L : 0;
+            const cx<double> p0 = half[m0];
+            v[e] = cmul(v[e], cx<T>{(T)p0.x, (T)p0.y});
+        } else {
+            int m = is + mc;
+            m = m < 0 ? m + 2 * L : (m >= 2 * L ? m - 2 * L : m);
+            const cx<double> ph = half[m];
+            w[e] = cmul(v[e], cx<T>{(T)ph.x, (T)ph.y});
+        }
     }
     fft_pencil<T, L, false, true, V>(v, q, lds, LdsMap{t, tkp}, tw);
     if (active) {
@@ -453,8 +470,8 @@ template <typename T> void fft_c2r(mfic_ctx ctx, T* data, const size_t n[3], dou
 
 // the two strided c2r passes only (axis 0, axis 1): the contiguous-axis pass
 // is left to a fused row kernel
-template <typename T, int L, int V>
-static void launch_ax0_shift_v(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc) {
+template <typename T, int L, int V, bool GEN>
+static void launch_ax0_shift_v(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc, int3 wa, int3 wb) {
     constexpr int P = Plan<L, V>::P;
     // at most 16 pencils per tile by default: at 512^3 FP32 the 16-pencil
     // tile takes 0.46 ms per launch, the 32-pencil tile 1.37 ms (two outputs
@@ -478,8 +495,8 @@ static void launch_ax0_shift_v(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1
 #define TKCASE(K)                                                                                               \
     case K:                                                                                                     \
         if constexpr (P * K <= 1024 && (V == 0 || K >= 4))                                                      \
-            hipLaunchKernelGGL((k_pencils_ax0_shift<T, L, K, V>), grid, dim3(P * K), lds, ctx->stream, data,    \
-                               shifted, n1, nc, (T)1, tw, half, K);                                             \
+            hipLaunchKernelGGL((k_pencils_ax0_shift<T, L, K, V, GEN>), grid, dim3(P * K), lds, ctx->stream,     \
+                               data, shifted, n1, nc, (T)1, tw, half, K, wa, wb);                               \
         break;
         TKCASE(1) TKCASE(2) TKCASE(4) TKCASE(8) TKCASE(16) TKCASE(32) TKCASE(64)
 #undef TKCASE
@@ -490,22 +507,33 @@ static void launch_ax0_shift_v(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1
 }
 
 template <typename T, int L>
-static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc) {
-    if (plan_variant(ctx, L, MFIC_PASS_AX0_SHIFT, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1)
-        launch_ax0_shift_v<T, L, alt_plan<L>()>(ctx, data, shifted, n1, nc);
-    else
-        launch_ax0_shift_v<T, L, 0>(ctx, data, shifted, n1, nc);
+static void launch_ax0_shift(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1, int nc, const int* wa,
+                             const int* wb) {
+    const int3 a = wa ? make_int3(wa[0], wa[1], wa[2]) : make_int3(0, 0, 0);
+    const int3 b = wb ? make_int3(wb[0], wb[1], wb[2]) : make_int3(1, 1, 1);
+    const bool gen = wa || wb;
+    if (plan_variant(ctx, L, MFIC_PASS_AX0_SHIFT, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 1) {
+        if (gen) launch_ax0_shift_v<T, L, alt_plan<L>(), true>(ctx, data, shifted, n1, nc, a, b);
+        else launch_ax0_shift_v<T, L, alt_plan<L>(), false>(ctx, data, shifted, n1, nc, a, b);
+    } else {
+        if (gen) launch_ax0_shift_v<T, L, 0, true>(ctx, data, shifted, n1, nc, a, b);
+        else launch_ax0_shift_v<T, L, 0, false>(ctx, data, shifted, n1, nc, a, b);
+    }
 }
 
-template <typename T> void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted, const size_t n[3]) {
+template <typename T>
+void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted, const size_t n[3], const int* wa, const int* wb) {
     check_sizes(n);
+    for (int a = 0; a < 3; ++a)
+        MFIC_CHECK((!wa || (wa[a] >= 0 && wa[a] <= 2)) && (!wb || (wb[a] >= 0 && wb[a] <= 2)), MFIC_ERR_ARG,
+                   "sub-lattice shift weights must be 0, 1 or 2 half cells");
     MFIC_CHECK(n[0] == n[1] && n[1] == n[2], MFIC_ERR_SIZE, "the shifted sub-lattice pass needs a cubic grid");
     const long nc = (long)(n[2] / 2 + 1);
     switch ((int)n[0]) {
 #define X(S)                                                                                                      \
     case S:                                                                                                       \
         launch_ax0_shift<T, S>(ctx, reinterpret_cast<cx<T>*>(data), reinterpret_cast<cx<T>*>(shifted), (int)n[1], \
-                               (int)nc);                                                                          \
+                               (int)nc, wa, wb);                                                                  \
         break;
         MFIC_FOR_SIZES(X)
 #undef X
@@ -515,8 +543,8 @@ template <typename T> void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted,
     for (T* d : {data, shifted})
         fft_pencils<T>(ctx, d, (int)n[1], nc, (long)n[1] * nc, (int)nc, (long)n[0], false, 1.0, "_ax1");
 }
-template void fft_c2r_cols_shift<float>(mfic_ctx, float*, float*, const size_t*);
-template void fft_c2r_cols_shift<double>(mfic_ctx, double*, double*, const size_t*);
+template void fft_c2r_cols_shift<float>(mfic_ctx, float*, float*, const size_t*, const int*, const int*);
+template void fft_c2r_cols_shift<double>(mfic_ctx, double*, double*, const size_t*, const int*, const int*);
 
 template <typename T> void fft_c2r_cols(mfic_ctx ctx, T* data, const size_t n[3]) {
     check_sizes(n);

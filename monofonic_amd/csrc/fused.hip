@@ -216,12 +216,13 @@ static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, i
 // with the lattice readout (particle_generator.hh:499-544, 597-639): each
 // group transforms one component row in LDS, then the row's particles are
 // written as interleaved FP64 [p][3], q + Psi for positions
-// ((ijk + shift)/N * lunit, grid_fft.hh:242-252) or Psi for velocities.
+// ((ijk + shift)/N * lunit per axis, grid_fft.hh:242-252, not wrapped) or Psi
+// for velocities.
 template <typename T, int M>
 __global__ void __launch_bounds__((fused_rows_per_block<M, 3>() * 3 * lane_slot<Plan<M>::P>()))
 k_rows_c2r_particles(const cx<T>* __restrict__ f0, const cx<T>* __restrict__ f1, const cx<T>* __restrict__ f2,
                      long nrows, T scale, const cx<T>* __restrict__ tw, double* __restrict__ dst, int is_pos,
-                     double lunit, double shift, size_t pbase) {
+                     double lunit, double3 shift, size_t pbase) {
     constexpr int E = Plan<M>::E, P = Plan<M>::P, SL = lane_slot<P>(), N = 2 * M;
     constexpr int RP = row_lds_pitch<M>(), RB = fused_rows_per_block<M, 3>(), TPR = 3 * SL;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -268,14 +269,15 @@ k_rows_c2r_particles(const cx<T>* __restrict__ f0, const cx<T>* __restrict__ f1,
         double val = (double)((p & 1) ? w.y : w.x);
         if (is_pos) {
             const int ijk = c == 0 ? i : (c == 1 ? j : p);
-            val = (((double)ijk + shift) / (double)N) * lunit + val;
+            const double sh = c == 0 ? shift.x : (c == 1 ? shift.y : shift.z);
+            val = (((double)ijk + sh) / (double)N) * lunit + val;
         }
         out[idx] = val;
     }
 }
 
 template <typename T, int M>
-static void launch_particles(mfic_ctx ctx, const void* const f[3], int is_pos, double lunit, double shift,
+static void launch_particles(mfic_ctx ctx, const void* const f[3], int is_pos, double lunit, const double shift[3],
                              size_t pbase, double* dst) {
     constexpr int SL = lane_slot<Plan<M>::P>(), RB = fused_rows_per_block<M, 3>(), RP = row_lds_pitch<M>();
     const long nrows = (long)(2 * M) * (2 * M);
@@ -289,14 +291,14 @@ static void launch_particles(mfic_ctx ctx, const void* const f[3], int is_pos, d
     hipLaunchKernelGGL((k_rows_c2r_particles<T, M>), grid, dim3(RB * 3 * SL), lds, ctx->stream,
                        static_cast<const cx<T>*>(f[0]), static_cast<const cx<T>*>(f[1]),
                        static_cast<const cx<T>*>(f[2]), nrows, (T)(1.0 / std::sqrt(N * N * N)), tw, dst, is_pos,
-                       lunit, shift, pbase);
+                       lunit, make_double3(shift[0], shift[1], shift[2]), pbase);
     MFIC_LAUNCH_CHECK();
 }
 
 #define MFIC_PARTICLE_ROWS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384)
 
-void rows_c2r_particles(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit, double shift,
-                        size_t pbase, double* dst) {
+void rows_c2r_particles(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit,
+                        const double shift[3], size_t pbase, double* dst) {
     switch (n / 2) {
 #define X(M)                                                                                  \
     case M:                                                                                   \

@@ -19,9 +19,10 @@ void fused_source_rows(mfic_ctx ctx, int prec, FusedKind kind, size_t n, const v
 int fused_inputs(FusedKind kind);
 // last (contiguous-axis) c2r pass of three component grids (k-space after the
 // axis-0 and axis-1 passes, [n][n][n/2+1] complex) fused with the lattice
-// readout into FP64 [p][3] at particle offset pbase
-void rows_c2r_particles(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit, double shift,
-                        size_t pbase, double* dst);
+// readout into FP64 [p][3] at particle offset pbase; positions sit at
+// (ijk + shift) cells per axis
+void rows_c2r_particles(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit,
+                        const double shift[3], size_t pbase, double* dst);
 int fused_outputs(FusedKind kind);
 
 }  // namespace mfic

@@ -41,6 +41,13 @@ void assemble_all(mfic_ctx ctx, int prec, size_t n, double L, int order, const d
 void lattice_readout3(mfic_ctx ctx, int prec, const void* const f[3], int n, int is_pos, double lunit, double shift,
                       size_t pbase, double* dst);
 
+// offset of the secondary lattice of a Bravais load in cells
+// (particle_generator.hh:58-63): sc (1/2,1/2,1/2), bcc (1/2,1/2,0),
+// fcc (1/2,1/2,1/2)
+inline void second_lattice_shift(int lattice, double g[3]) {
+    g[0] = g[1] = 0.5;
+    g[2] = lattice == MFIC_LATTICE_BCC ? 0.0 : 0.5;
+}
 void lattice_readout_raw(mfic_ctx ctx, int prec, const void* field, int n, int idim, int is_pos, double lunit,
                          double shift, size_t pbase, double* dst);
 

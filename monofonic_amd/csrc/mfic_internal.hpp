@@ -112,6 +112,11 @@ class ProfScope {
 // scratch buffer of at least `bytes`, keyed by name, owned by the context
 void* ctx_scratch(mfic_ctx ctx, const std::string& key, size_t bytes);
 
+// refuses (MFIC_ERR_ARG) a caller pointer that kernels would access but that
+// is not device or managed memory: a kernel store to pageable host memory
+// faults the device
+void require_device_memory(const void* p, const char* what);
+
 // twiddles: complex transform of length L (kind 0), or real rows of length
 // 2M (kind 1: complex length M table followed by M+1 untangle factors), or
 // the half-cell shift phases exp(i pi m / L), m < 2L (kind 2, FP64); the
@@ -133,8 +138,12 @@ bool fft_length_supported(size_t n);
 // pass-level entry points used by fused pipelines
 template <typename T> void fft_c2r_cols(mfic_ctx ctx, T* data, const size_t n[3]);
 // fft_c2r_cols of a cubic-grid component plus, from the same axis-0 loads,
-// the component of the (1/2,1/2,1/2)-cell shifted sub-lattice into `shifted`
-template <typename T> void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted, const size_t n[3]);
+// the component of the (1/2,1/2,1/2)-cell shifted sub-lattice into `shifted`;
+// with weights (wa, wb not null) the in-place output is shifted by wa / 2
+// and the second by wb / 2 cells per axis (0, 1 or 2 each)
+template <typename T>
+void fft_c2r_cols_shift(mfic_ctx ctx, T* data, T* shifted, const size_t n[3], const int* wa = nullptr,
+                        const int* wb = nullptr);
 template <typename T> void fft_rows_r2c(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
 template <typename T> void fft_rows_c2r(mfic_ctx ctx, T* data, size_t nrows, int n2, double scale);
 // complex pencils of length L: element (o, p, t) at data + o*os + p*ps + t,

@@ -216,6 +216,7 @@ struct GroupSpec {
     bool long_ids;          // uint64 ParticleIDs
     bool filtered;          // chunked + Fletcher-32 datasets
     std::vector<std::pair<const char*, double>> constants;  // [N] datasets of one value (Masses, ...)
+    bool const_f64 = true;  // the constant datasets as 64-bit (else 32-bit) floats
 };
 
 // Coordinates, Velocities, ParticleIDs and the constant datasets of one
@@ -235,8 +236,10 @@ static void write_particle_group(Hdf5& h, hid_t file, const ParticleSource& src,
     H5Obj dids(create_dataset(h, g.id, "ParticleIDs", idtype, idsize, isp.id, false, N, gs.filtered), h.Dclose,
                "dataset ParticleIDs");
     std::vector<std::unique_ptr<H5Obj>> dconst;
+    const hid_t ctype = gs.const_f64 ? h.DOUBLE : h.FLOAT;
     for (const auto& c : gs.constants)
-        dconst.emplace_back(new H5Obj(create_dataset(h, g.id, c.first, h.DOUBLE, 8, isp.id, false, N, gs.filtered),
+        dconst.emplace_back(new H5Obj(create_dataset(h, g.id, c.first, ctype, gs.const_f64 ? 8 : 4, isp.id, false, N,
+                                                     gs.filtered),
                                       h.Dclose, c.first));
 
     // chunk by chunk: [pos | vel | ids] per chunk
@@ -340,7 +343,9 @@ static void write_particle_group(Hdf5& h, hid_t file, const ParticleSource& src,
             H5CK(h.Dwrite(dvel.id, vtype, vmem.id, vsp.id, 0, hb + vbytes), "write Velocities");
             H5CK(h.Dwrite(dids.id, idtype, imem.id, isp.id, 0, hb + 2 * vbytes), "write ParticleIDs");
             for (size_t k = 0; k < gs.constants.size(); ++k) {
-                std::fill(cvals.begin(), cvals.begin() + cnt, gs.constants[k].second);
+                // the value as the container holds it (mass32_ or mass64_), converted by HDF5
+                const double v = gs.const_f64 ? gs.constants[k].second : (double)(float)gs.constants[k].second;
+                std::fill(cvals.begin(), cvals.begin() + cnt, v);
                 H5CK(h.Dwrite(dconst[k]->id, h.DOUBLE, imem.id, isp.id, 0, cvals.data()), gs.constants[k].first);
             }
             // the next stage into this buffer must wait for these writes:
@@ -361,7 +366,7 @@ static void write_ics_parameters(Hdf5& h, hid_t file, const mfic_ic_config& cfg,
     H5Obj g(h.Gcreate2(file, "ICs_parameters", 0, 0, 0), h.Gclose, "group ICs_parameters");
     static const char* loads[] = {"sc", "bcc", "fcc"};
     const int order = cfg.lpt_order, fix = cfg.do_fixing != 0, inv = cfg.do_inversion != 0, zero = 0,
-              L = (int)cfg.n;
+              bar = cfg.baryons != 0, L = (int)cfg.n;
     attr_string(h, g.id, "Code", "monofonic_amd (MI355X)");
     attr_string(h, g.id, "Git Revision", mfic_version());
     attr_string(h, g.id, "Git Tag", mfic_version());
@@ -376,13 +381,13 @@ static void write_ics_parameters(Hdf5& h, hid_t file, const mfic_ic_config& cfg,
     attr_string(h, g.id, "Random Generator", "NGENIC");
     attr_scalar(h, g.id, "Mode Fixing", h.INT, &fix);
     attr_scalar(h, g.id, "Mode inversion", h.INT, &inv);
-    attr_scalar(h, g.id, "Baryons", h.INT, &zero);
+    attr_scalar(h, g.id, "Baryons", h.INT, &bar);
     attr_scalar(h, g.id, "Baryons Relative Velocity", h.INT, &zero);
     attr_scalar(h, g.id, "Grid Resolution", h.INT, &L);
 }
 
 void gadget_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_params& cosmo,
-                  const mfic_gadget_opts& opts, const ParticleSource& src, bool tabulated) {
+                  const mfic_gadget_opts& opts, const ParticleSource& src, const ParticleSource* bsrc, bool tabulated) {
     MFIC_CHECK(fname && *fname, MFIC_ERR_ARG, "null file name");
     Hdf5& h = hdf5();
     mfic_cosmo_scalars sc;
@@ -395,22 +400,39 @@ void gadget_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo
     // unlink + HDFCreateFile: H5F_ACC_TRUNC replaces an existing file
     H5Obj file(h.Fcreate(fname, 0x0002u, 0, 0), h.Fclose, (std::string("create ") + fname).c_str());
     write_ics_parameters(h, file.id, cfg, tabulated);
+    // species masses (ic_generator.cc:157-166, no neutrino exclusion)
+    const bool bar = bsrc != nullptr;
+    const double Omega_dm = bar ? cosmo.Omega_m - cosmo.Omega_b : cosmo.Omega_m;
+    const size_t Nb = bar ? bsrc->npart : 0;
     // PartType1 (dark matter, get_species_idx) with Coordinates, Velocities
-    // and ParticleIDs (output_gadget_hdf5.cc:268-318)
-    write_particle_group(h, file.id, src, GroupSpec{"PartType1", false, opts.long_ids != 0, false, {}});
+    // and ParticleIDs (output_gadget_hdf5.cc:268-318); with baryons every
+    // particle carries its mass (set_masses, ic_generator.cc:747-778), so
+    // both species add Masses in the plugin's precision and the MassTable
+    // stays zero
+    GroupSpec dm{"PartType1", false, opts.long_ids != 0, false, {}};
+    if (bar) {
+        dm.constants.push_back({"Masses", N ? Omega_dm * munit / (double)N : 0.0});
+        dm.const_f64 = opts.double_precision != 0;
+    }
+    write_particle_group(h, file.id, src, dm);
+    if (bar) {
+        GroupSpec gb{"PartType0", false, opts.long_ids != 0, false, {{"Masses", Nb ? cosmo.Omega_b * munit / (double)Nb : 0.0}}};
+        gb.const_f64 = opts.double_precision != 0;
+        write_particle_group(h, file.id, *bsrc, gb);
+    }
     {
         // Header (output_gadget_hdf5.cc:175-210)
         H5Obj g(h.Gcreate2(file.id, "Header", 0, 0, 0), h.Gclose, "group Header");
-        const double Omega_dm = cosmo.Omega_m;  // no baryons, no neutrino exclusion (ic_generator.cc:156-195)
         double mass[6] = {0, 0, 0, 0, 0, 0};
-        mass[1] = N ? Omega_dm * munit / (double)N : 0.0;
+        if (!bar) mass[1] = N ? Omega_dm * munit / (double)N : 0.0;
         if (opts.gadget2_compat) {
-            unsigned np[6] = {0, (unsigned)N, 0, 0, 0, 0}, hw[6] = {0, (unsigned)((uint64_t)N >> 32), 0, 0, 0, 0};
+            unsigned np[6] = {(unsigned)Nb, (unsigned)N, 0, 0, 0, 0},
+                     hw[6] = {(unsigned)((uint64_t)Nb >> 32), (unsigned)((uint64_t)N >> 32), 0, 0, 0, 0};
             attr_array(h, g.id, "NumPart_ThisFile", h.UINT, np, 6);
             attr_array(h, g.id, "NumPart_Total", h.UINT, np, 6);
             attr_array(h, g.id, "NumPart_Total_HighWord", h.UINT, hw, 6);
         } else {
-            unsigned long long np[6] = {0, (unsigned long long)N, 0, 0, 0, 0};
+            unsigned long long np[6] = {(unsigned long long)Nb, (unsigned long long)N, 0, 0, 0, 0};
             attr_array(h, g.id, "NumPart_ThisFile", h.ULLONG, np, 6);
             attr_array(h, g.id, "NumPart_Total", h.ULLONG, np, 6);
         }
@@ -434,7 +456,7 @@ void gadget_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo
 }
 
 void swift_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_params& cosmo,
-                 const mfic_swift_opts& opts, const ParticleSource& src, bool tabulated) {
+                 const mfic_swift_opts& opts, const ParticleSource& src, const ParticleSource* bsrc, bool tabulated) {
     MFIC_CHECK(fname && *fname, MFIC_ERR_ARG, "null file name");
     Hdf5& h = hdf5();
     mfic_cosmo_scalars sc;
@@ -455,19 +477,43 @@ void swift_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_
         attr_scalar(h, g.id, "Unit temperature in cgs (U_T)", h.DOUBLE, &one);
     }
     write_ics_parameters(h, file.id, cfg, tabulated);
-    // PartType1 with per-particle Masses (output_swift.cc:250-377): the
-    // container carries no individual masses, so every particle has
-    // Omega_dm munit / N (Omega_dm = Omega_m without baryons)
-    const double pmass = N ? cosmo.Omega_m * munit / (double)N : 0.0;
+    // PartType1 with per-particle Masses (output_swift.cc:250-377): every
+    // particle has Omega_s munit / N_s, whether the container holds the
+    // masses (with baryons) or the plugin fills them in
+    const bool bar = bsrc != nullptr;
+    const double Omega_dm = bar ? cosmo.Omega_m - cosmo.Omega_b : cosmo.Omega_m;
+    const size_t Nb = bar ? bsrc->npart : 0;
+    const double pmass = N ? Omega_dm * munit / (double)N : 0.0;
     write_particle_group(h, file.id, src, GroupSpec{"PartType1", true, opts.long_ids != 0, true, {{"Masses", pmass}}});
+    if (bar) {
+        // PartType0: gas with the initial internal energy and the mean
+        // inter-particle separation as smoothing length (output_swift.cc:
+        // 86-110, 297-302, 360-366)
+        const double astart = 1.0 / (1.0 + cosmo.zstart), hh = sc.h;
+        const double gamma = opts.gamma > 0 ? opts.gamma : 5.0 / 3.0, YHe = opts.YHe > 0 ? opts.YHe : 0.245421;
+        const double Tcmb0 = cosmo.Tcmb;
+        const double npol = (std::fabs(1.0 - gamma) > 1e-7) ? 1.0 / (gamma - 1.) : 1.0;
+        const double unitv = 1e5;
+        const double adec = 1.0 / (160. * std::pow(cosmo.Omega_b * hh * hh / 0.022, 2.0 / 5.0));
+        const double Tini = astart < adec ? Tcmb0 / astart : Tcmb0 / astart / astart * adec;
+        const double mu = (Tini > 1.e4) ? 4.0 / (8. - 5. * YHe) : 4.0 / (1. + 3. * (1. - YHe));
+        const double ceint = 1.3806e-16 / 1.6726e-24 * Tini * npol / mu / unitv / unitv;
+        const double hsml = cfg.L / hh / (double)cfg.n;
+        write_particle_group(h, file.id, *bsrc,
+                             GroupSpec{"PartType0", true, opts.long_ids != 0, true,
+                                       {{"Masses", Nb ? cosmo.Omega_b * munit / (double)Nb : 0.0},
+                                        {"InternalEnergy", ceint},
+                                        {"SmoothingLength", hsml}}});
+    }
     {
         // Header, written by the plugin's destructor (output_swift.cc:183-200);
         // its MassTable stays zero (masses are per particle)
         H5Obj g(h.Gcreate2(file.id, "Header", 0, 0, 0), h.Gclose, "group Header");
         const int dim = 3, izero = 0, one = 1;
         const double box = cfg.L / sc.h, time = 1.0 / (1.0 + cosmo.zstart), redshift = cosmo.zstart;
-        unsigned tot[7] = {0, (unsigned)N, 0, 0, 0, 0, 0}, hw[7] = {0, (unsigned)((uint64_t)N >> 32), 0, 0, 0, 0, 0};
-        unsigned long long thisfile[7] = {0, (unsigned long long)N, 0, 0, 0, 0, 0};
+        unsigned tot[7] = {(unsigned)Nb, (unsigned)N, 0, 0, 0, 0, 0},
+                 hw[7] = {(unsigned)((uint64_t)Nb >> 32), (unsigned)((uint64_t)N >> 32), 0, 0, 0, 0, 0};
+        unsigned long long thisfile[7] = {(unsigned long long)Nb, (unsigned long long)N, 0, 0, 0, 0, 0};
         const double mass[7] = {0, 0, 0, 0, 0, 0, 0};
         attr_scalar(h, g.id, "Dimension", h.INT, &dim);
         attr_scalar(h, g.id, "BoxSize", h.DOUBLE, &box);

@@ -20,14 +20,18 @@ struct ParticleSource {
 
 // output_plugin units of the Gadget-HDF5 writer (output_gadget_hdf5.cc:79-85)
 void gadget_units(double boxlength, double zstart, double* lunit, double* vunit, double* munit);
-// one Gadget-HDF5 file of a run's particles; tabulated: the run used a
-// tabulated transfer function (recorded in ICs_parameters)
+// one Gadget-HDF5 file of a run's particles; baryons: the baryon species
+// (secondary lattice) or null; tabulated: the run used a tabulated transfer
+// function (recorded in ICs_parameters)
 void gadget_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_params& cosmo,
-                  const mfic_gadget_opts& opts, const ParticleSource& src, bool tabulated);
+                  const mfic_gadget_opts& opts, const ParticleSource& src, const ParticleSource* baryons,
+                  bool tabulated);
 // output_plugin units of the SWIFT writer (output_swift.cc:63-69)
 void swift_units(double boxlength, const mfic_cosmo_params& cosmo, double* lunit, double* vunit, double* munit);
-// one SWIFT initial-condition file of a run's particles
+// one SWIFT initial-condition file of a run's particles; baryons: the
+// baryon species (secondary lattice) or null
 void swift_write(const char* fname, const mfic_ic_config& cfg, const mfic_cosmo_params& cosmo,
-                 const mfic_swift_opts& opts, const ParticleSource& src, bool tabulated);
+                 const mfic_swift_opts& opts, const ParticleSource& src, const ParticleSource* baryons,
+                 bool tabulated);
 
 }  // namespace mfic

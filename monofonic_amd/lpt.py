@@ -211,6 +211,7 @@ class ICConfig:
     inversion: bool = False
     lunit: float = 1.0
     vunit: float = 1.0
+    baryons: bool = False
 
     def to_c(self) -> L.IcConfig:
         c = L.IcConfig()
@@ -218,7 +219,7 @@ class ICConfig:
             int(self.plt), self.prec
         c.do_fixing, c.do_inversion, c.seed, c.particles = int(self.fixing), int(self.inversion), self.seed, \
             int(self.particles)
-        c.lunit, c.vunit = self.lunit, self.vunit
+        c.lunit, c.vunit, c.baryons = self.lunit, self.vunit, int(self.baryons)
         return c
 
 
@@ -247,14 +248,17 @@ class ICRun:
         L.call("mfic_run_potential", self.h, which, C.byref(g))
         return Grid(self.ctx, (self.cfg.n,) * 3, (self.cfg.box,) * 3, prec=self.cfg.prec, handle=g)
 
-    def particles_device(self):
+    def particles_device(self, baryons: bool = False):
+        """device pointers of the dark matter (or, with baryons=True, the
+        baryon species') FP64 [p][3] positions and velocities, and npart"""
         pos, vel, n = C.c_void_p(), C.c_void_p(), C.c_size_t()
-        L.call("mfic_run_particles", self.h, C.byref(pos), C.byref(vel), C.byref(n))
+        L.call("mfic_run_baryon_particles" if baryons else "mfic_run_particles", self.h, C.byref(pos), C.byref(vel),
+               C.byref(n))
         return pos.value, vel.value, n.value
 
-    def particles(self):
+    def particles(self, baryons: bool = False):
         """download FP64 positions and velocities, shape (npart, 3) each"""
-        pos, vel, n = self.particles_device()
+        pos, vel, n = self.particles_device(baryons)
         out = []
         for ptr in (pos, vel):
             host = np.empty(3 * n, dtype=np.float64)

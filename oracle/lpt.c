@@ -344,19 +344,26 @@ void orc_shift_field(int n, double L, const double s[3], double *f) {
 }
 
 static const double SHIFTS[2][2][3] = {{{0, 0, 0}, {0, 0, 0}}, {{0, 0, 0}, {0.5, 0.5, 0.5}}};
+/* second_lattice_shift (particle_generator.hh:58-63): sc, bcc */
+static const double SECOND[2][3] = {{0.5, 0.5, 0.5}, {0.5, 0.5, 0.0}};
 
-static void readout(int n, double L, int lattice, int idim, double lunit, int is_pos,
+/* set_positions / set_velocities (particle_generator.hh:499-544, 597-639)
+ * for lattice index 0 or 1: index 1 shifts the field by the secondary
+ * lattice's offset before the first sub-lattice and offsets q by it */
+static void readout(int n, double L, int lattice, int lattice_index, int idim, double lunit, int is_pos,
                     double *f, double *outv) {
     const size_t np1 = (size_t)n * n * n;
     const int nshift = 1 << lattice;
     for (int is = 0; is < nshift; ++is) {
+        if (is == 0 && lattice_index == 1) orc_shift_field(n, L, SECOND[lattice], f);
         if (is > 0) {
             double s[3] = {SHIFTS[lattice][is][0] - SHIFTS[lattice][is - 1][0],
                            SHIFTS[lattice][is][1] - SHIFTS[lattice][is - 1][1],
                            SHIFTS[lattice][is][2] - SHIFTS[lattice][is - 1][2]};
             orc_shift_field(n, L, s, f);
         }
-        const double *sh = SHIFTS[lattice][is];
+        double sh[3];
+        for (int a = 0; a < 3; ++a) sh[a] = SHIFTS[lattice][is][a] + (lattice_index == 1 ? SECOND[lattice][a] : 0.0);
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j)
@@ -375,11 +382,15 @@ static void readout(int n, double L, int lattice, int idim, double lunit, int is
 
 void orc_lattice_set_positions(int n, double L, int lattice, int idim, double lunit,
                                double *field, double *pos) {
-    readout(n, L, lattice, idim, lunit, 1, field, pos);
+    readout(n, L, lattice, 0, idim, lunit, 1, field, pos);
 }
 void orc_lattice_set_velocities(int n, double L, int lattice, int idim,
                                 double *field, double *vel) {
-    readout(n, L, lattice, idim, 0.0, 0, field, vel);
+    readout(n, L, lattice, 0, idim, 0.0, 0, field, vel);
+}
+void orc_lattice_readout_index(int n, double L, int lattice, int lattice_index, int idim, double lunit,
+                               int is_pos, double *field, double *out) {
+    readout(n, L, lattice, lattice_index, idim, lunit, is_pos, field, out);
 }
 
 /* ------------------------------------------------------ full run ------- */

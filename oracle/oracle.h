@@ -181,6 +181,10 @@ void orc_lpt_component_k(const orc_ic_cfg *cfg, const orc_lpt_coef *k,
  * 499-544, 597-639) for sc / bcc, lattice_index 0.  `field` is the real-space
  * component (A1 layout) and is modified (phase shifted) for bcc.
  * out has 3*n^3*overload doubles interleaved [ipart][3]. */
+/* set_positions / set_velocities of lattice index 0 or 1 (the secondary
+ * lattice of the baryon species), particle_generator.hh:499-544, 597-639 */
+void orc_lattice_readout_index(int n, double L, int lattice, int lattice_index, int idim, double lunit,
+                               int is_pos, double *field, double *out);
 void orc_lattice_set_positions(int n, double L, int lattice, int idim, double lunit,
                                double *field, double *pos);
 void orc_lattice_set_velocities(int n, double L, int lattice, int idim,

@@ -130,6 +130,8 @@ def lib():
         L.orc_zero_dc.argtypes = [_dp]
         L.orc_scale.argtypes = [C.c_int, _dp, C.c_double]
         L.orc_lattice_set_positions.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp]
+        L.orc_lattice_readout_index.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                C.c_int, _dp, _dp]
         L.orc_lattice_set_velocities.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp]
         L.orc_ic_run.argtypes = [C.POINTER(IcCfg), C.POINTER(Cosmo), C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p]
@@ -368,6 +370,17 @@ def lattice_readout(field: np.ndarray, L: float, lattice: int, idim: int, is_pos
     else:
         lib().orc_lattice_set_velocities(n, L, lattice, idim, field, out)
     return out.reshape(-1, 3)
+
+
+def lattice_readout_index(field: np.ndarray, L: float, lattice: int, lattice_index: int, idim: int, is_pos: bool,
+                          lunit: float = 1.0):
+    """readout of one component (real-space field, modified in place as the
+    reference modifies it) on lattice index 0 or 1; (npart, 3) array with
+    column idim set"""
+    n = field.shape[0]
+    out = np.zeros(((1 << lattice) * n ** 3, 3))
+    lib().orc_lattice_readout_index(n, L, lattice, lattice_index, idim, lunit, int(is_pos), field, out)
+    return out
 
 
 def plt_tables(lattice: int, ng: int = 64) -> Plt:

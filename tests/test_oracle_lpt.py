@@ -102,3 +102,47 @@ def test_readout_sc_and_bcc():
     got = pos2[3 * n ** 3::3].reshape(n, n, n) - ((i + 0.5) / n) * lunit
     assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
     assert np.array_equal(pos2[: 3 * n ** 3: 3].reshape(n, n, n), (i / n) * lunit + base)
+
+
+def _series(spec, n, L, pts):
+    """the band-limited field's Fourier series at points (npts, 3) [length]"""
+    m = np.arange(n)
+    sm = np.where(m > n // 2, m - n, m)
+    kk = np.arange(n // 2 + 1)
+    kf = 2 * math.pi / L
+    KX, KY, KZ = np.meshgrid(sm * kf, sm * kf, kk * kf, indexing="ij")
+    w = np.where(KZ > 0, 2.0, 1.0)  # Hermitian partners of kz > 0
+    out = np.zeros(len(pts))
+    for p, xyz in enumerate(pts):
+        e = np.exp(1j * (KX * xyz[0] + KY * xyz[1] + KZ * xyz[2]))
+        out[p] = np.real(np.sum(w * spec * e)) / n ** 1.5
+    return out
+
+
+@pytest.mark.parametrize("lattice", [0, 1])
+def test_secondary_lattice_readout(lattice):
+    """lattice index 1 (the baryon species, particle_generator.hh:58-63,
+    512-531): every sub-lattice s reads the field's Fourier series at
+    (ijk + second + s/2) dx and places q there, with second = (1/2,1/2,1/2)
+    for sc and (1/2,1/2,0) for bcc (not wrapped: sub-lattice 1 of bcc sits at
+    (ijk + (1, 1, 1/2)) dx); a planted offset (the sc shift on bcc) fails"""
+    n, L, lunit = 8, 20.0, 3.0
+    f, spec = _band_limited(n, 12 + lattice)
+    base = O.rview(f).copy()
+    second = np.array([0.5, 0.5, 0.5]) if lattice == 0 else np.array([0.5, 0.5, 0.0])
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ijk = np.stack([i, j, k]).reshape(3, -1).T
+    dx = L / n
+    for idim in range(3):
+        g = O.empty_field(n)
+        O.rview(g)[...] = base
+        out = O.lattice_readout_index(g, L, lattice, 1, idim, True, lunit)
+        for s in range(1 << lattice):
+            off = second + 0.5 * s
+            want = _series(spec, n, L, (ijk + off) * dx)
+            got = out[s * n ** 3:(s + 1) * n ** 3, idim]
+            q = (ijk[:, idim] + off[idim]) / n * lunit
+            assert np.max(np.abs(got - q - want)) < 1e-12 * np.max(np.abs(want)), (idim, s)
+            if lattice == 1 and s == 0 and idim == 2:
+                wrong = _series(spec, n, L, (ijk + 0.5) * dx)
+                assert np.max(np.abs(got - q - wrong)) > 1e-3 * np.max(np.abs(want))
