@@ -285,19 +285,20 @@ int mfic_lpt_component(mfic_ctx ctx, int order, int comp, const mfic_lpt_coef *k
                        mfic_grid A3y, mfic_grid A3z, mfic_grid out);
 
 /* ----------------------------------------------------------------- readout --- */
-/* lattice_generator::set_positions / set_velocities for Bravais sc / bcc,
- * lattice index 0 (particle_generator.hh:499-544, 597-639): `field` is the
+/* lattice_generator::set_positions / set_velocities for Bravais sc / bcc /
+ * fcc, lattice index 0 (particle_generator.hh:499-544, 597-639): `field` is the
  * real-space component; positions are ((ijk + shift)/N) * lunit + field in
  * FP64.  dst is device memory for 3*npart doubles (interleaved [ipart][3];
  * host memory is refused with MFIC_ERR_ARG); is_pos = 0 writes velocities
- * (lunit ignored).  For bcc the
- * second sub-lattice is read after a Fourier shift by (1/2,1/2,1/2) cells
- * (the field is left shifted, as the reference leaves it). */
+ * (lunit ignored).  Every further sub-lattice (bcc (1/2,1/2,1/2); fcc
+ * (0,1/2,1/2), (1/2,0,1/2), (1/2,1/2,0)) is read after a Fourier shift by its
+ * offset from the previous one (the field is left shifted, as the reference
+ * leaves it). */
 int mfic_lattice_readout(mfic_grid field, int lattice, int idim, int is_pos, double lunit,
                          double *dst_device);
 /* the same for lattice index 0 or 1: index 1 (the secondary lattice of the
- * baryon species) first shifts the field by second_lattice_shift (sc
- * (1/2,1/2,1/2), bcc (1/2,1/2,0); particle_generator.hh:58-63, 512-531)
+ * baryon species) first shifts the field by second_lattice_shift (sc and
+ * fcc (1/2,1/2,1/2), bcc (1/2,1/2,0); particle_generator.hh:58-63, 512-531)
  * and offsets the positions by it */
 int mfic_lattice_readout_index(mfic_grid field, int lattice, int lattice_index, int idim, int is_pos,
                                double lunit, double *dst);
@@ -307,7 +308,7 @@ typedef struct {
     size_t n;               /* GridRes                                       */
     double L;               /* BoxLength [Mpc/h]                             */
     int lpt_order;          /* 1..3 (LPTorder)                               */
-    int lattice;            /* mfic_lattice (ParticleLoad sc / bcc)          */
+    int lattice;            /* mfic_lattice (ParticleLoad sc / bcc / fcc)    */
     int plt;                /* PLT corrections (ENABLE_PLT)                  */
     int prec;               /* bulk precision MFIC_F32 / MFIC_F64            */
     int do_fixing, do_inversion;
@@ -335,8 +336,8 @@ int mfic_run_particles(mfic_run r, double **pos, double **vel, size_t *npart);
  * the same LPT displacements and velocities (delta_bc = theta_bc = 0 with
  * the Eisenstein & Hu fit, so C_species has no effect) read out on the
  * secondary lattice, offset by second_lattice_shift (particle_generator.hh:
- * 58-63: sc (1/2,1/2,1/2), bcc (1/2,1/2,0)); sub-lattice s at that offset plus
- * s (1/2,1/2,1/2), positions not wrapped.  Device pointers, [p][3] FP64,
+ * 58-63: sc and fcc (1/2,1/2,1/2), bcc (1/2,1/2,0)); every sub-lattice at that
+ * offset plus its own, positions not wrapped.  Device pointers, [p][3] FP64,
  * npart as for the dark matter.  The dark matter takes Omega_m - Omega_b,
  * the baryons Omega_b (ic_generator.cc:157-166). */
 int mfic_run_baryon_particles(mfic_run r, double **pos, double **vel, size_t *npart);
@@ -442,7 +443,7 @@ typedef struct {
  * rho_crit BoxLength^3 [1e10 Msun/h]; a run written with mfic_gadget_write
  * must use the first two as lunit and vunit */
 int mfic_gadget_units(double boxlength, double zstart, double *lunit, double *vunit, double *munit);
-/* write_particle_data + the header of a run with particles = 1 (sc or bcc),
+/* write_particle_data + the header of a run with particles = 1 (sc, bcc, fcc),
  * streamed chunk by chunk from its device arrays (the run must have
  * executed); cosmological header values come from the run's parameters.  A
  * run with baryons adds /PartType0 (IDs after the dark matter's) and, as the

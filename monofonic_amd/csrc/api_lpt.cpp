@@ -39,7 +39,7 @@ struct mfic_run_s {
     mfic::EhParams eh{};
     mfic_lpt_coef coef{};
     mfic_grid pot[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // phi1 phi2 phi3 A3x A3y A3z
-    mfic_grid comp[12] = {};  // particle mode: component fields per sub-lattice
+    mfic_grid comp[24] = {};  // particle mode: component fields per sub-lattice (fcc: 4)
     mfic_grid fields[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t* d_table = nullptr;
     void* d_tk = nullptr;  // spline nodes of a tabulated transfer function
@@ -283,8 +283,8 @@ int mfic_lattice_readout_index(mfic_grid field, int lattice, int lattice_index, 
     return guard_call([&] {
         need(field, "field");
         MFIC_CHECK(dst, MFIC_ERR_ARG, "null destination");
-        MFIC_CHECK(lattice == MFIC_LATTICE_SC || lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
-                   "readout supports sc and bcc lattices");
+        MFIC_CHECK(lattice >= MFIC_LATTICE_SC && lattice <= MFIC_LATTICE_FCC, MFIC_ERR_UNSUPPORTED,
+                   "readout supports the sc, bcc and fcc lattices");
         MFIC_CHECK(lattice_index == 0 || lattice_index == 1, MFIC_ERR_ARG, "lattice index must be 0 or 1");
         MFIC_HIP(hipSetDevice(field->ctx->device));
         require_device_memory(dst, "the readout destination");
@@ -298,13 +298,20 @@ int mfic_lattice_readout_index(mfic_grid field, int lattice, int lattice_index, 
             int rc = mfic_grid_shift(field, g, 1);
             if (rc != MFIC_OK) throw Error(rc, mfic_last_error());
         }
-        lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, g[idim], 0, dst);
-        if (lattice == MFIC_LATTICE_BCC) {
-            const double s[3] = {0.5, 0.5, 0.5};
-            int rc = mfic_grid_shift(field, s, 1);
-            if (rc != MFIC_OK) throw Error(rc, mfic_last_error());
-            lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit, 0.5 + g[idim],
-                                (size_t)n * n * n, dst);
+        // lattice_shifts (particle_generator.hh:47-53); each sub-lattice is
+        // read after a shift relative to the previous one, as the reference does
+        static const double SUB[3][4][3] = {{{0, 0, 0}},
+                                            {{0, 0, 0}, {0.5, 0.5, 0.5}},
+                                            {{0, 0, 0}, {0, 0.5, 0.5}, {0.5, 0, 0.5}, {0.5, 0.5, 0}}};
+        for (int sub = 0; sub < (1 << lattice); ++sub) {
+            if (sub > 0) {
+                double rel[3];
+                for (int a = 0; a < 3; ++a) rel[a] = SUB[lattice][sub][a] - SUB[lattice][sub - 1][a];
+                int rc = mfic_grid_shift(field, rel, 1);
+                if (rc != MFIC_OK) throw Error(rc, mfic_last_error());
+            }
+            lattice_readout_raw(field->ctx, field->prec, field->d, n, idim, is_pos, lunit,
+                                SUB[lattice][sub][idim] + g[idim], (size_t)sub * n * n * n, dst);
         }
     });
 }
@@ -317,8 +324,8 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
     return guard_call([&] {
         MFIC_CHECK(ctx && cfg && cosmo && out, MFIC_ERR_ARG, "null argument");
         MFIC_CHECK(cfg->lpt_order >= 1 && cfg->lpt_order <= 3, MFIC_ERR_CONFIG, "LPTorder must be 1, 2 or 3");
-        MFIC_CHECK(cfg->lattice == MFIC_LATTICE_SC || cfg->lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
-                   "ParticleLoad must be sc or bcc");
+        MFIC_CHECK(cfg->lattice >= MFIC_LATTICE_SC && cfg->lattice <= MFIC_LATTICE_FCC, MFIC_ERR_UNSUPPORTED,
+                   "ParticleLoad must be sc, bcc or fcc");
         MFIC_CHECK(cfg->prec == MFIC_F32 || cfg->prec == MFIC_F64, MFIC_ERR_CONFIG, "bad precision");
         MFIC_CHECK(cfg->n % 2 == 0 || cfg->lpt_order == 1, MFIC_ERR_SIZE,
                    "Orszag convolver for LPTorder>1 requires even grid size!");
@@ -346,7 +353,9 @@ int mfic_run_create(mfic_ctx ctx, const mfic_ic_config* cfg, const mfic_cosmo_pa
         for (int p = 0; p < (cfg->lpt_order == 1 ? 1 : cfg->lpt_order == 2 ? 2 : 6); ++p)
             r->pot[p] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
         if (cfg->particles) {
-            const int ncomp = (cfg->lattice == MFIC_LATTICE_BCC || cfg->baryons) ? 12 : 6;
+            // component grids: 6 per sub-lattice, and a second set for the
+            // shifted output of the sc baryon pass
+            const int ncomp = cfg->lattice == MFIC_LATTICE_FCC ? 24 : (cfg->lattice == MFIC_LATTICE_BCC || cfg->baryons) ? 12 : 6;
             for (int q = 0; q < ncomp; ++q) r->comp[q] = new_grid(ctx, n, cfg->L, cfg->prec, MFIC_KSPACE);
             r->npart = (size_t(1) << cfg->lattice) * n * n * n;
             r->d_pos = static_cast<double*>(dmalloc(3 * r->npart * sizeof(double), "particle positions"));
@@ -411,52 +420,63 @@ int mfic_run_execute(mfic_run r, int blocking) {
         // components come out of the first sub-lattice's axis-0 pass
         assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3, outs, 1);
         if (c.particles) {
-            // strided passes per component, then the contiguous pass fused
-            // with the readout of the three components of each sub-lattice
-            for (int q = 0; q < 6; ++q) {
-                if (nsub > 1) {
-                    if (c.prec == MFIC_F64) fft_c2r_cols_shift<double>(r->ctx, (double*)og[q]->d, (double*)og[q + 6]->d, nn);
-                    else fft_c2r_cols_shift<float>(r->ctx, (float*)og[q]->d, (float*)og[q + 6]->d, nn);
-                } else {
-                    if (c.prec == MFIC_F64) fft_c2r_cols<double>(r->ctx, (double*)og[q]->d, nn);
-                    else fft_c2r_cols<float>(r->ctx, (float*)og[q]->d, nn);
-                }
-            }
-            for (int s = 0; s < nsub; ++s) {
-                const void* fp[3] = {og[6 * s]->d, og[6 * s + 1]->d, og[6 * s + 2]->d};
-                const void* fv[3] = {og[6 * s + 3]->d, og[6 * s + 4]->d, og[6 * s + 5]->d};
-                const size_t pbase = (size_t)s * c.n * c.n * c.n;
-                const double sh[3] = {0.5 * s, 0.5 * s, 0.5 * s};
-                rows_c2r_particles(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, sh, pbase, r->d_pos);
-                rows_c2r_particles(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, sh, pbase, r->d_vel);
-            }
-            if (c.baryons) {
-                // the baryon species on the secondary lattice (ic_generator.cc:
-                // 716-745, particle_generator.hh:512-531): with delta_bc = 0
-                // (Eisenstein & Hu) its displacements and velocities are the
-                // same fields read out at the lattice shifted by
-                // second_lattice_shift, sub-lattice s at second + s/2 cells.
-                // The components are assembled again (the dark matter pass
-                // transformed them in place) and each axis-0 pass forms both
-                // sub-lattices' phases from one load.
-                assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3,
-                             outs, 1);
-                double g[3];
-                second_lattice_shift(c.lattice, g);
-                const int wa[3] = {(int)(2 * g[0]), (int)(2 * g[1]), (int)(2 * g[2])};
-                const int wb[3] = {wa[0] + 1, wa[1] + 1, wa[2] + 1};
+            // sub-lattice offsets in half cells (lattice_shifts,
+            // particle_generator.hh:47-53)
+            static const int SUB[3][4][3] = {{{0, 0, 0}},
+                                             {{0, 0, 0}, {1, 1, 1}},
+                                             {{0, 0, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}}};
+            const size_t cbytes = og[0]->bytes();
+            // strided passes per component, every sub-lattice's phase formed
+            // in the axis-0 pass from one load per output pair, then the
+            // contiguous pass fused with the readout of the three components
+            // of each sub-lattice; gw = global offset of the species' lattice
+            auto species = [&](const int gw[3], double* dpos, double* dvel) {
+                const bool global = gw[0] || gw[1] || gw[2];
                 for (int q = 0; q < 6; ++q) {
-                    if (c.prec == MFIC_F64) fft_c2r_cols_shift<double>(r->ctx, (double*)og[q]->d, (double*)og[q + 6]->d, nn, wa, wb);
-                    else fft_c2r_cols_shift<float>(r->ctx, (float*)og[q]->d, (float*)og[q + 6]->d, nn, wa, wb);
+                    if (nsub == 1 && !global) {
+                        if (c.prec == MFIC_F64) fft_c2r_cols<double>(r->ctx, (double*)og[q]->d, nn);
+                        else fft_c2r_cols<float>(r->ctx, (float*)og[q]->d, nn);
+                        continue;
+                    }
+                    int w[4][3];
+                    for (int s = 0; s < 4; ++s)
+                        for (int a = 0; a < 3; ++a) w[s][a] = gw[a] + SUB[c.lattice][s < nsub ? s : 0][a];
+                    if (nsub == 4)  // fcc: the second pair of sub-lattices from a copy of the component
+                        MFIC_HIP(hipMemcpyAsync(og[q + 12]->d, og[q]->d, cbytes, hipMemcpyDeviceToDevice, r->ctx->stream));
+                    for (int pr = 0; pr < (nsub == 4 ? 2 : 1); ++pr) {
+                        void* a = og[q + 12 * pr]->d;
+                        void* b = og[q + 12 * pr + 6]->d;
+                        const int* wa = (pr == 0 && !global && nsub == 2) ? nullptr : w[2 * pr];
+                        const int* wb = (pr == 0 && !global && nsub == 2) ? nullptr : w[2 * pr + 1];
+                        if (c.prec == MFIC_F64) fft_c2r_cols_shift<double>(r->ctx, (double*)a, (double*)b, nn, wa, wb);
+                        else fft_c2r_cols_shift<float>(r->ctx, (float*)a, (float*)b, nn, wa, wb);
+                    }
                 }
                 for (int s = 0; s < nsub; ++s) {
                     const void* fp[3] = {og[6 * s]->d, og[6 * s + 1]->d, og[6 * s + 2]->d};
                     const void* fv[3] = {og[6 * s + 3]->d, og[6 * s + 4]->d, og[6 * s + 5]->d};
                     const size_t pbase = (size_t)s * c.n * c.n * c.n;
-                    const double sh[3] = {g[0] + 0.5 * s, g[1] + 0.5 * s, g[2] + 0.5 * s};
-                    rows_c2r_particles(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, sh, pbase, r->d_pos_b);
-                    rows_c2r_particles(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, sh, pbase, r->d_vel_b);
+                    double sh[3];
+                    for (int a = 0; a < 3; ++a) sh[a] = 0.5 * (gw[a] + SUB[c.lattice][s][a]);
+                    rows_c2r_particles(r->ctx, c.prec, fp, (int)c.n, 1, c.lunit, sh, pbase, dpos);
+                    rows_c2r_particles(r->ctx, c.prec, fv, (int)c.n, 0, 0.0, sh, pbase, dvel);
                 }
+            };
+            const int zero[3] = {0, 0, 0};
+            species(zero, r->d_pos, r->d_vel);
+            if (c.baryons) {
+                // the baryon species on the secondary lattice (ic_generator.cc:
+                // 716-745, particle_generator.hh:512-531): with delta_bc = 0
+                // (Eisenstein & Hu) its displacements and velocities are the
+                // same fields read out on the lattice offset by
+                // second_lattice_shift.  The components are assembled again:
+                // the dark matter pass transformed them in place.
+                assemble_all(r->ctx, c.prec, c.n, c.L, c.lpt_order, cd, cv, c.lunit / c.L, c.vunit / c.L, pv, phi, A3,
+                             outs, 1);
+                double g[3];
+                second_lattice_shift(c.lattice, g);
+                const int gw[3] = {(int)(2 * g[0]), (int)(2 * g[1]), (int)(2 * g[2])};
+                species(gw, r->d_pos_b, r->d_vel_b);
             }
         } else {
             for (int q = 0; q < 6; ++q) {
@@ -554,8 +574,8 @@ int mfic_gadget_write_host(const char* fname, const mfic_ic_config* cfg, const m
                            const mfic_gadget_opts* opts, const double* pos, const double* vel) {
     return guard_call([&] {
         MFIC_CHECK(fname && cfg && cosmo && opts && pos && vel, MFIC_ERR_ARG, "null argument");
-        MFIC_CHECK(cfg->lattice == MFIC_LATTICE_SC || cfg->lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
-                   "ParticleLoad must be sc or bcc");
+        MFIC_CHECK(cfg->lattice >= MFIC_LATTICE_SC && cfg->lattice <= MFIC_LATTICE_FCC, MFIC_ERR_UNSUPPORTED,
+                   "ParticleLoad must be sc, bcc or fcc");
         MFIC_CHECK(cfg->n > 0, MFIC_ERR_SIZE, "empty lattice");
         check_gadget_units(*cfg, cosmo->zstart);
         ParticleSource src;
@@ -607,8 +627,8 @@ int mfic_swift_write_host(const char* fname, const mfic_ic_config* cfg, const mf
                           const mfic_swift_opts* opts, const double* pos, const double* vel) {
     return guard_call([&] {
         MFIC_CHECK(fname && cfg && cosmo && opts && pos && vel, MFIC_ERR_ARG, "null argument");
-        MFIC_CHECK(cfg->lattice == MFIC_LATTICE_SC || cfg->lattice == MFIC_LATTICE_BCC, MFIC_ERR_UNSUPPORTED,
-                   "ParticleLoad must be sc or bcc");
+        MFIC_CHECK(cfg->lattice >= MFIC_LATTICE_SC && cfg->lattice <= MFIC_LATTICE_FCC, MFIC_ERR_UNSUPPORTED,
+                   "ParticleLoad must be sc, bcc or fcc");
         MFIC_CHECK(cfg->n > 0, MFIC_ERR_SIZE, "empty lattice");
         check_swift_units(*cfg, *cosmo);
         ParticleSource src;

@@ -343,9 +343,12 @@ void orc_shift_field(int n, double L, const double s[3], double *f) {
     orc_fft3d_c2r(f, n, n, n);
 }
 
-static const double SHIFTS[2][2][3] = {{{0, 0, 0}, {0, 0, 0}}, {{0, 0, 0}, {0.5, 0.5, 0.5}}};
-/* second_lattice_shift (particle_generator.hh:58-63): sc, bcc */
-static const double SECOND[2][3] = {{0.5, 0.5, 0.5}, {0.5, 0.5, 0.0}};
+/* lattice_shifts (particle_generator.hh:47-53): sc, bcc, fcc */
+static const double SHIFTS[3][4][3] = {{{0, 0, 0}},
+                                       {{0, 0, 0}, {0.5, 0.5, 0.5}},
+                                       {{0, 0, 0}, {0, 0.5, 0.5}, {0.5, 0, 0.5}, {0.5, 0.5, 0}}};
+/* second_lattice_shift (particle_generator.hh:58-63): sc, bcc, fcc */
+static const double SECOND[3][3] = {{0.5, 0.5, 0.5}, {0.5, 0.5, 0.0}, {0.5, 0.5, 0.5}};
 
 /* set_positions / set_velocities (particle_generator.hh:499-544, 597-639)
  * for lattice index 0 or 1: index 1 shifts the field by the secondary

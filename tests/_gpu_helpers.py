@@ -42,15 +42,19 @@ def read_rows(ctx, dptr, rows, width, dtype=np.float64):
     return out
 
 
+SUBLATTICES = {L.SC: [(0, 0, 0)], L.BCC: [(0, 0, 0), (0.5, 0.5, 0.5)],
+               L.FCC: [(0, 0, 0), (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)]}
+
+
 def lattice_q(n, lattice, particles, lunit=1.0):
     """Lagrangian positions ((ijk + shift) / n) * lunit of particle indices
     (particle_generator.hh:47-63, 499-544; sub-lattice s holds indices
-    s n^3 .. (s+1) n^3 - 1, bcc shift (1/2, 1/2, 1/2))"""
+    s n^3 .. (s+1) n^3 - 1 at the lattice_shifts offset s)"""
     p = np.asarray(particles, dtype=np.int64)
     s = p // n ** 3
     c = p % n ** 3
     ijk = np.stack(np.unravel_index(c, (n, n, n)), axis=1).astype(np.float64)
-    shift = 0.5 * s[:, None] if lattice == L.BCC else 0.0
+    shift = np.array(SUBLATTICES[lattice], dtype=np.float64)[s]
     return (ijk + shift) / n * lunit
 
 

@@ -1,4 +1,4 @@
-"""Baryon + CDM particle runs on the device (ic_generator.cc:704-778 with the
+"""fcc particle runs and baryon + CDM particle runs on the device (ic_generator.cc:704-778 with the
 Eisenstein & Hu fit, where delta_bc = theta_bc = 0): the dark matter is the
 single-species run, the baryons read the same displacement and velocity
 fields out on the secondary lattice (particle_generator.hh:58-63, 499-544,
@@ -17,20 +17,20 @@ import pytest
 from monofonic_amd import F32, F64, Grid, _lib as L, lpt
 from oracle import oracle as O
 
-from _gpu_helpers import max_rel, rel_l2
+from _gpu_helpers import SUBLATTICES, max_rel, rel_l2
 
 TOL = {F64: 1e-10, F32: 1e-5}
 TOL_MAX = {F64: 1e-9, F32: 1e-4}
-SECOND = {L.SC: (0.5, 0.5, 0.5), L.BCC: (0.5, 0.5, 0.0)}
+SECOND = {L.SC: (0.5, 0.5, 0.5), L.BCC: (0.5, 0.5, 0.0), L.FCC: (0.5, 0.5, 0.5)}
 
 
-def secondary_q(n, lattice, lunit):
-    """Lagrangian positions of the secondary lattice, sub-lattice s at
-    (ijk + second + s/2) / n * lunit, not wrapped"""
+def species_q(n, lattice, lunit, index):
+    """Lagrangian positions of lattice index 0 or 1 (the secondary lattice),
+    sub-lattice s at (ijk + shift_s + second) / n * lunit, not wrapped"""
     i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     ijk = np.stack([i, j, k]).reshape(3, -1).T.astype(np.float64)
-    return np.concatenate([(ijk + np.array(SECOND[lattice]) + 0.5 * s) / n * lunit
-                           for s in range(1 << lattice)])
+    g = np.array(SECOND[lattice]) if index else np.zeros(3)
+    return np.concatenate([(ijk + np.array(sub) + g) / n * lunit for sub in SUBLATTICES[lattice]])
 
 
 def oracle_species(cfg, cos):
@@ -51,7 +51,32 @@ def oracle_species(cfg, cos):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", [F64, F32])
-@pytest.mark.parametrize("lattice,plt,order", [(L.SC, False, 2), (L.BCC, False, 3), (L.BCC, True, 3)])
+@pytest.mark.parametrize("plt", [False, True])
+def test_fcc_run_matches_oracle(ctx, prec, plt):
+    """ParticleLoad fcc: four sub-lattices (0,0,0), (0,1/2,1/2), (1/2,0,1/2),
+    (1/2,1/2,0), with and without the fcc PLT corrections, 3LPT"""
+    n, box, z, seed = 32, 48.0, 24.0, 1618
+    print(f"fcc: n={n} plt={plt} prec={prec}", flush=True)
+    cfg = O.ic_cfg(n, box, 3, seed, lattice=L.FCC, plt=int(plt), lunit=box, vunit=2.0)
+    _, pos, vel, _ = O.ic_run(cfg, O.baseline_cosmology(z), particles=True)
+    run = lpt.ICRun(ctx, lpt.ICConfig(n=n, box=box, zstart=z, lpt_order=3, lattice=L.FCC, plt=plt, prec=prec,
+                                      seed=seed, particles=True, lunit=box, vunit=2.0))
+    run.execute()
+    p, v = run.particles()
+    run.close()
+    assert len(p) == 4 * n ** 3
+    q = species_q(n, L.FCC, box, 0)
+    pos, vel = pos.reshape(-1, 3), vel.reshape(-1, 3)
+    e, m = rel_l2(p - q, pos - q), max_rel(p - q, pos - q)
+    ev, mv = rel_l2(v, vel), max_rel(v, vel)
+    print(f"  Psi rel-L2 {e:.2e} max {m:.2e}; v rel-L2 {ev:.2e} max {mv:.2e}", flush=True)
+    assert e < TOL[prec] and m < TOL_MAX[prec] and ev < TOL[prec] and mv < TOL_MAX[prec]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [F64, F32])
+@pytest.mark.parametrize("lattice,plt,order", [(L.SC, False, 2), (L.BCC, False, 3), (L.BCC, True, 3),
+                                               (L.FCC, False, 2)])
 def test_baryon_run_matches_oracle(ctx, prec, lattice, plt, order):
     n, box, z, seed = 32, 48.0, 24.0, 2718
     print(f"baryons: n={n} lattice={lattice} plt={plt} order={order} prec={prec}", flush=True)
@@ -63,10 +88,8 @@ def test_baryon_run_matches_oracle(ctx, prec, lattice, plt, order):
     p, v = run.particles()
     pb, vb = run.particles(baryons=True)
     run.close()
-    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
-    ijk = np.stack([i, j, k]).reshape(3, -1).T.astype(np.float64)
-    q0 = np.concatenate([(ijk + 0.5 * s) / n * box for s in range(1 << lattice)])
-    qb = secondary_q(n, lattice, box)
+    q0 = species_q(n, lattice, box, 0)
+    qb = species_q(n, lattice, box, 1)
     for name, got, want, q in (("dm", p, dpos, q0), ("baryon", pb, bpos, qb)):
         e, m = rel_l2(got - q, want - q), max_rel(got - q, want - q)
         print(f"  {name} Psi rel-L2 {e:.2e} max {m:.2e}", flush=True)
@@ -82,10 +105,10 @@ def test_baryon_run_matches_oracle(ctx, prec, lattice, plt, order):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lattice", [L.SC, L.BCC])
+@pytest.mark.parametrize("lattice,index", [(L.SC, 1), (L.BCC, 1), (L.FCC, 0), (L.FCC, 1)])
 @pytest.mark.parametrize("is_pos", [1, 0])
-def test_lattice_readout_index_matches_oracle(ctx, lattice, is_pos):
-    """mfic_lattice_readout_index on lattice index 1 against the oracle's"""
+def test_lattice_readout_index_matches_oracle(ctx, lattice, index, is_pos):
+    """mfic_lattice_readout_index against the oracle's readout"""
     n, box, lunit = 16, 20.0, 3.0
     rng = np.random.default_rng(31 + lattice)
     a = O.empty_field(n)
@@ -98,12 +121,12 @@ def test_lattice_readout_index_matches_oracle(ctx, lattice, is_pos):
     torch = pytest.importorskip("torch")
     npart = (1 << lattice) * n ** 3
     for idim in range(3):
-        want = O.lattice_readout_index(a.copy(), box, lattice, 1, idim, bool(is_pos), lunit)[:, idim]
+        want = O.lattice_readout_index(a.copy(), box, lattice, index, idim, bool(is_pos), lunit)[:, idim]
         g = Grid(ctx, (n,) * 3, (box,) * 3, prec=F64)
         g.upload(a)
         dst = torch.zeros(3 * npart, dtype=torch.float64, device="cuda")  # device memory, as the entry point needs
         torch.cuda.synchronize()
-        L.call("mfic_lattice_readout_index", g.h, lattice, 1, idim, is_pos, lunit, C.c_void_p(dst.data_ptr()))
+        L.call("mfic_lattice_readout_index", g.h, lattice, index, idim, is_pos, lunit, C.c_void_p(dst.data_ptr()))
         ctx.sync()
         out = dst.cpu().numpy().reshape(-1, 3)
         g.close()
@@ -112,7 +135,7 @@ def test_lattice_readout_index_matches_oracle(ctx, lattice, is_pos):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lattice", [L.SC, L.BCC])
+@pytest.mark.parametrize("lattice", [L.SC, L.BCC, L.FCC])
 def test_baryon_particle_files(ctx, tmp_path, lattice):
     """Gadget-HDF5 (PartType0 / PartType1, per-particle Masses, zero
     MassTable) and SWIFT (PartType0 with InternalEnergy and SmoothingLength)

@@ -159,7 +159,7 @@ def check_cubic_symmetry(tab, ng, lattice):
     h = ng // 2
     ints = np.rint(K).astype(int)
     ok = np.all(np.abs(ints) < h, axis=-1) & np.any(ints != 0, axis=-1)
-    if lattice == 1:
+    if lattice >= 1:
         ok &= np.abs(ints).sum(axis=-1) > 1
     sites = np.argwhere(ok)
     worst = 0.0
@@ -180,7 +180,7 @@ def check_cubic_symmetry(tab, ng, lattice):
     return worst
 
 
-@pytest.mark.parametrize("lattice", [0, 1])
+@pytest.mark.parametrize("lattice", [0, 1, 2])
 def test_plt_tables_cubic_symmetry(lattice):
     ng = 16
     p = O.plt_tables(lattice, ng)

@@ -119,30 +119,37 @@ def _series(spec, n, L, pts):
     return out
 
 
-@pytest.mark.parametrize("lattice", [0, 1])
-def test_secondary_lattice_readout(lattice):
-    """lattice index 1 (the baryon species, particle_generator.hh:58-63,
-    512-531): every sub-lattice s reads the field's Fourier series at
-    (ijk + second + s/2) dx and places q there, with second = (1/2,1/2,1/2)
-    for sc and (1/2,1/2,0) for bcc (not wrapped: sub-lattice 1 of bcc sits at
-    (ijk + (1, 1, 1/2)) dx); a planted offset (the sc shift on bcc) fails"""
+SUBS = {0: [(0, 0, 0)], 1: [(0, 0, 0), (0.5, 0.5, 0.5)],
+        2: [(0, 0, 0), (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)]}
+SECOND = {0: (0.5, 0.5, 0.5), 1: (0.5, 0.5, 0.0), 2: (0.5, 0.5, 0.5)}
+
+
+@pytest.mark.parametrize("lattice", [0, 1, 2])
+@pytest.mark.parametrize("index", [0, 1])
+def test_lattice_readout_series(lattice, index):
+    """lattice index 0 and 1 (the baryon species, particle_generator.hh:
+    47-63, 499-544): sub-lattice s reads the field's Fourier series at
+    (ijk + shift_s + second) dx and places q there, shift_s the sc / bcc /
+    fcc sub-lattice offsets, second = (1/2,1/2,1/2) for sc and fcc and
+    (1/2,1/2,0) for bcc on index 1 (positions not wrapped); a planted offset
+    (the sc shift on bcc) fails"""
     n, L, lunit = 8, 20.0, 3.0
-    f, spec = _band_limited(n, 12 + lattice)
+    f, spec = _band_limited(n, 12 + lattice + 3 * index)
     base = O.rview(f).copy()
-    second = np.array([0.5, 0.5, 0.5]) if lattice == 0 else np.array([0.5, 0.5, 0.0])
+    second = np.array(SECOND[lattice]) if index else np.zeros(3)
     i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     ijk = np.stack([i, j, k]).reshape(3, -1).T
     dx = L / n
     for idim in range(3):
         g = O.empty_field(n)
         O.rview(g)[...] = base
-        out = O.lattice_readout_index(g, L, lattice, 1, idim, True, lunit)
-        for s in range(1 << lattice):
-            off = second + 0.5 * s
+        out = O.lattice_readout_index(g, L, lattice, index, idim, True, lunit)
+        for s, sub in enumerate(SUBS[lattice]):
+            off = second + np.array(sub)
             want = _series(spec, n, L, (ijk + off) * dx)
             got = out[s * n ** 3:(s + 1) * n ** 3, idim]
             q = (ijk[:, idim] + off[idim]) / n * lunit
             assert np.max(np.abs(got - q - want)) < 1e-12 * np.max(np.abs(want)), (idim, s)
-            if lattice == 1 and s == 0 and idim == 2:
+            if lattice == 1 and index == 1 and s == 0 and idim == 2:
                 wrong = _series(spec, n, L, (ijk + 0.5) * dx)
                 assert np.max(np.abs(got - q - wrong)) > 1e-3 * np.max(np.abs(want))

@@ -18,7 +18,8 @@ from oracle import oracle as O
 NG = 16
 PI = math.pi
 REC = {0: np.diag([2 * PI] * 3),
-       1: np.array([[2 * PI, 0, 0], [0, 2 * PI, 0], [-2 * PI, -2 * PI, 4 * PI]])}
+       1: np.array([[2 * PI, 0, 0], [0, 2 * PI, 0], [-2 * PI, -2 * PI, 4 * PI]]),
+       2: np.array([[-4 * PI, 4 * PI, 2 * PI], [0, 0, 2 * PI], [4 * PI, 0, -2 * PI]])}
 INVREC = {0: np.diag([2, 2, 2]), 1: np.array([[2, 0, 0], [0, 2, 0], [1, 1, 1]])}
 NORMALS = {0: [(PI, 0, 0), (-PI, 0, 0), (0, PI, 0), (0, -PI, 0), (0, 0, PI), (0, 0, -PI)],
            1: [(0, PI, PI), (0, -PI, PI), (0, PI, -PI), (0, -PI, -PI), (PI, 0, PI), (-PI, 0, PI),
@@ -38,7 +39,7 @@ def matrices(lat):
     return M
 
 
-@pytest.mark.parametrize("lat", [0, 1])
+@pytest.mark.parametrize("lat", [0, 1, 2])
 def test_dynamical_matrix_sum_rule_and_small_k(lat):
     M = matrices(lat)
     tr = np.trace(M, axis1=-2, axis2=-1)
