@@ -171,6 +171,9 @@ MFIC_PLAN(768, 24, 3, 12, 8, 8, 0)
 // alternatives of the padded-pass lengths: 16 threads per transform
 MFIC_PLAN_V(512, 1, 32, 3, 16, 8, 4, 0)
 MFIC_PLAN_V(768, 1, 48, 3, 16, 12, 4, 0)
+// row plan (V 2) for the output transforms of the 16-input fused pass: 32
+// threads per 384-point row
+MFIC_PLAN_V(384, 2, 12, 4, 12, 4, 4, 2)
 #undef MFIC_PLAN
 #undef MFIC_PLAN_V
 // number of radix plans compiled for a length
@@ -178,6 +181,8 @@ template <int L> constexpr int plan_variants() { return (L == 512 || L == 768) ?
 // plan 1 of a length that has one, else plan 0 (lets run-time dispatch name
 // plan 1 for every length without instantiating a missing plan)
 template <int L> constexpr int alt_plan() { return plan_variants<L>() > 1 ? 1 : 0; }
+// the row plan 2 of a length that has one, else plan 0
+template <int L> constexpr int row_alt_plan() { return L == 384 ? 2 : 0; }
 
 template <int L, int V = 0> constexpr int plan_ns(int s) {  // span before stage s
     int ns = 1;
@@ -373,13 +378,17 @@ __device__ __forceinline__ void fft_pencil(cx<T>* v, int q, cx<T>* lds, LdsMap m
 
 // full transform of a row held by one thread group, exchanges through the
 // per-stage swizzled LDS slots starting at `base`
-template <typename T, int L, bool FWD, class SYNC = BlockSync>
+template <typename T, int L, bool FWD, class SYNC = BlockSync, int V = 0>
 __device__ __forceinline__ void fft_regs_row(cx<T>* v, int q, cx<T>* lds, int base, const cx<T>* __restrict__ tw) {
-    constexpr int S = Plan<L>::S;
-    fft_stage<T, L, 0, FWD, LdsSwzMap<L, 0>, SYNC>(v, q, lds, LdsSwzMap<L, 0>{base}, tw);
-    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, LdsSwzMap<L, 1 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 1 % S>{base}, tw);
-    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, LdsSwzMap<L, 2 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 2 % S>{base}, tw);
-    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, LdsSwzMap<L, 3 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 3 % S>{base}, tw);
+    constexpr int S = Plan<L, V>::S;
+    if constexpr (V != 0) {  // row plans other than the default: plain row addressing
+        fft_regs<T, L, FWD, LdsMap, SYNC, false, V>(v, q, lds, LdsMap{base, 1}, tw);
+    } else {
+        fft_stage<T, L, 0, FWD, LdsSwzMap<L, 0>, SYNC>(v, q, lds, LdsSwzMap<L, 0>{base}, tw);
+        if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, LdsSwzMap<L, 1 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 1 % S>{base}, tw);
+        if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, LdsSwzMap<L, 2 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 2 % S>{base}, tw);
+        if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, LdsSwzMap<L, 3 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 3 % S>{base}, tw);
+    }
 }
 
 }  // namespace mfic

@@ -26,6 +26,7 @@ namespace mfic {
 // 2LPT source (ic_generator.cc:536-541)
 struct ProdSrc2 {
     static constexpr int NI = 6, NO = 1, IPG = 1, WAVES = 3;
+    static constexpr bool OUT32 = false;
     __device__ static __forceinline__ void eval(const double* h, double* o) {
         o[0] = h[0] * (h[1] + h[2]) + h[1] * h[2] - h[3] * h[3] - h[4] * h[4] - h[5] * h[5];
     }
@@ -35,6 +36,7 @@ struct ProdSrc2 {
 // (ic_generator.cc:581-585)
 struct ProdSrc2Q {
     static constexpr int NI = 6, NO = 5, IPG = 1, WAVES = 3;
+    static constexpr bool OUT32 = false;
     __device__ static __forceinline__ void eval(const double* h, double* o) {
         ProdSrc2::eval(h, o);
         o[1] = h[0] * h[1] - h[3] * h[3];
@@ -50,6 +52,9 @@ struct ProdSrc3 {
     // two inputs per lane group: 4-wave workgroups; registers for two waves
     // per SIMD (no spills), i.e. two workgroups per CU
     static constexpr int NI = 16, NO = 4, IPG = 1, WAVES = 2;
+    // the four output transforms on 32 threads each (two waves) instead of
+    // 16 (one wave): the output phase was a third of the pass
+    static constexpr bool OUT32 = true;
     __device__ static __forceinline__ void eval(const double* h, double* o) {
         const double* a = h;
         const double* b = h + 6;
@@ -76,7 +81,7 @@ template <typename T, int M, class PR>
 __global__ void __launch_bounds__((fused_rows_per_block<M, PR::NI / PR::IPG>() * (PR::NI / PR::IPG) * lane_slot<Plan<M>::P>()))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PR::WAVES : 1, 8)))
 k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out, long ostride, int kc, long nrows,
-             T scale, const cx<T>* __restrict__ tw) {
+             T scale, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ tw2) {
     constexpr int NI = PR::NI, NO = PR::NO, IPG = PR::IPG, NG = NI / IPG;
     constexpr int E = Plan<M>::E, P = Plan<M>::P, SL = lane_slot<P>();
     constexpr int RP = row_lds_pitch<M>(), RB = fused_rows_per_block<M, NG>(), TPR = NG * SL;
@@ -167,7 +172,36 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
     __syncthreads();
 
     // 3. r2c of outputs g, g + NG, ... < NO, first kc half-spectrum values written
-    if (lane) {
+    constexpr int RV = row_alt_plan<M>();
+    constexpr bool O32 = PR::OUT32 && RV == 2 && RB == 1 && NO * Plan<M, RV>::P <= NG * SL;
+    if constexpr (O32) {
+        constexpr int E2 = Plan<M, RV>::E, P2 = Plan<M, RV>::P;
+        const int o = tr / P2, q2 = tr % P2;
+        const cx<T>* uw2 = tw2 + plan_tw_size<M, RV>();
+        if (o < NO) {
+            const int base = (rb * NI + o) * RP;
+            cx<T> v[E2];
+#pragma unroll
+            for (int e = 0; e < E2; ++e) v[e] = lds[base + q2 + P2 * e];
+            WaveSync::sync();
+            fft_regs_row<T, M, true, WaveSync, RV>(v, q2, lds, base, tw2);
+#pragma unroll
+            for (int e = 0; e < E2; ++e) lds[base + q2 + P2 * e] = v[e];
+            WaveSync::sync();
+            if (active) {
+                cx<T>* dst = out + o * ostride + row * (long)kc;
+#pragma unroll
+                for (int e = 0; e < E2; ++e) {
+                    const int k = q2 + P2 * e;
+                    if (k >= kc) continue;
+                    const cx<T> zk = v[e], zm = lds[base + (k == 0 ? 0 : M - k)];
+                    const cx<T> Ek = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+                    const cx<T> Ok = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+                    dst[k] = cadd(Ek, cmul(uw2[k], Ok));
+                }
+            }
+        }
+    } else if (lane) {
         for (int o = g; o < NO; o += NG) {
             const LdsRowMap lm{(rb * NI + o) * RP};
             cx<T> v[E];
@@ -203,12 +237,16 @@ static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, i
     const size_t lds = (size_t)RB * NI * RP * sizeof(cx<T>);
     MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
+    // the output transforms' own plan (row plan 2) where the pass uses one
+    const cx<T>* tw2 = (PR::OUT32 && row_alt_plan<M>() == 2 && RB == 1)
+                           ? static_cast<const cx<T>*>(twiddles(ctx, M, 1, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, 2))
+                           : tw;
     dim3 grid((unsigned)xcd_grid((nrows + RB - 1) / RB));
     ProfScope prof(ctx, std::string(name) + "_n" + std::to_string(2 * M) + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    (double)nrows * (NI + NO) * kc * sizeof(cx<T>));
     hipLaunchKernelGGL((k_rows_fused<T, M, PR>), grid, dim3(RB * NG * SL), lds, ctx->stream,
                        static_cast<const cx<T>*>(in), stride, static_cast<cx<T>*>(out), stride, kc, nrows, (T)scale,
-                       tw);
+                       tw, tw2);
     MFIC_LAUNCH_CHECK();
 }
 

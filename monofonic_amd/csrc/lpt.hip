@@ -28,52 +28,69 @@ __device__ __forceinline__ double fgrad(int idx, int n, double kf) {
 }
 
 // k-space CIC on an ng x ng x (ng/2+1) table (Grid_FFT::get_cic_kspace,
-// grid_fft.hh:347-373: periodic in x and y, clamped at the last z plane)
-__device__ inline double cic_k(const double* __restrict__ T, int ng, double x, double y, double z) {
+// grid_fft.hh:347-373: periodic in x and y, clamped at the last z plane).
+// The eight corners and weights are formed once per mode and applied to
+// every table.
+struct CicStencil {
+    unsigned off[8];
+    double w[8];
+};
+__device__ inline void cic_stencil(int ng, double x, double y, double z, CicStencil& s) {
     const int nc = ng / 2 + 1;
     const int ix = (int)floor(x), iy = (int)floor(y), iz = (int)floor(z);
     const double dx = x - ix, tx = 1.0 - dx, dy = y - iy, ty = 1.0 - dy, dz = z - iz, tz = 1.0 - dz;
     const int ix1 = (ix + 1) % ng, iy1 = (iy + 1) % ng, iz1 = min(iz + 1, nc - 1);
-    auto at = [&](int a, int b, int c) { return T[((size_t)a * ng + b) * nc + c]; };
+    auto at = [&](int a, int b, int c) { return (unsigned)((a * ng + b) * nc + c); };
+    s.off[0] = at(ix, iy, iz);    s.w[0] = tx * ty * tz;
+    s.off[1] = at(ix, iy, iz1);   s.w[1] = tx * ty * dz;
+    s.off[2] = at(ix, iy1, iz);   s.w[2] = tx * dy * tz;
+    s.off[3] = at(ix, iy1, iz1);  s.w[3] = tx * dy * dz;
+    s.off[4] = at(ix1, iy, iz);   s.w[4] = dx * ty * tz;
+    s.off[5] = at(ix1, iy, iz1);  s.w[5] = dx * ty * dz;
+    s.off[6] = at(ix1, iy1, iz);  s.w[6] = dx * dy * tz;
+    s.off[7] = at(ix1, iy1, iz1); s.w[7] = dx * dy * dz;
+}
+__device__ inline double cic_apply(const double* __restrict__ T, const CicStencil& s) {
     double v = 0.0;
-    v += at(ix, iy, iz) * tx * ty * tz;
-    v += at(ix, iy, iz1) * tx * ty * dz;
-    v += at(ix, iy1, iz) * tx * dy * tz;
-    v += at(ix, iy1, iz1) * tx * dy * dz;
-    v += at(ix1, iy, iz) * dx * ty * tz;
-    v += at(ix1, iy, iz1) * dx * ty * dz;
-    v += at(ix1, iy1, iz) * dx * dy * tz;
-    v += at(ix1, iy1, iz1) * dx * dy * dz;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v += T[s.off[c]] * s.w[c];
     return v;
 }
 
 // lattice_gradient::gradient for all three axes + vfac_corr
-// (particle_plt.hh:551-585), for N-grid mode (i, j, k) of a box of length L
-__device__ inline void plt_grad(const PltDev& P, int n, double L, int i, int j, int k, double g[3], double* vcorr) {
+// (particle_plt.hh:551-585), for N-grid mode (i, j, k); mr = ng / n and
+// c0 = 1 / (mr L) come from the host
+__device__ inline void plt_grad(const PltDev& P, double mr, double c0, int i, int j, int k, double g[3],
+                                double* vcorr) {
     const int ng = P.ng;
-    const double mr = (double)ng / (double)n;
     const double ix = i * mr, iy = j * mr, iz = k * mr;
     const double kf = 2.0 * 3.14159265358979323846;
     const double hn = (double)(ng / 2);
     const double kx = (ix - (double)(ix > hn) * ng) * kf, ky = (iy - (double)(iy > hn) * ng) * kf,
                  kz = (iz - (double)(iz > hn) * ng) * kf;
-    const double kr = sqrt(kx * kx + ky * ky + kz * kz);
-    const double kmod = kr / mr / L;
-    const double Dr = cic_k(P.Dr, ng, ix, iy, iz), Dt = cic_k(P.Dtheta, ng, ix, iy, iz),
-                 Dp = cic_k(P.Dphi, ng, ix, iy, iz);
+    const double kp2 = kx * kx + ky * ky;
+    const double kr = sqrt(kp2 + kz * kz);
+    const double kmod = kr * c0;
+    CicStencil cs;
+    cic_stencil(ng, ix, iy, iz, cs);
+    const double Dr = cic_apply(P.Dr, cs), Dt = cic_apply(P.Dtheta, cs), Dp = cic_apply(P.Dphi, cs);
     // spherical basis of k without trigonometry: theta = acos(kz/|k|),
     // phi = atan2(ky, kx) (phi = 0 on the kz axis, as atan2(0, 0))
     double st = 0.0, ct = 1.0, sp = 0.0, cp = 1.0;
     if (kr > 0.0) {
-        const double kp = sqrt(kx * kx + ky * ky);
-        ct = kz / kr;
-        st = kp / kr;
-        if (kp > 0.0) { cp = kx / kp; sp = ky / kp; }
+        const double kp = sqrt(kp2), ir = 1.0 / kr;
+        ct = kz * ir;
+        st = kp * ir;
+        if (kp > 0.0) {
+            const double ip = 1.0 / kp;
+            cp = kx * ip;
+            sp = ky * ip;
+        }
     }
     g[0] = kmod * (Dr * st * cp + Dt * ct * cp - Dp * sp);
     g[1] = kmod * (Dr * st * sp + Dt * ct * sp + Dp * cp);
     g[2] = kmod * (Dr * ct - Dt * st);
-    if (vcorr) *vcorr = cic_k(P.vfac, ng, ix, iy, iz);
+    if (vcorr) *vcorr = cic_apply(P.vfac, cs);
 }
 
 template <typename T>
@@ -87,7 +104,8 @@ __global__ void k_assemble(cx<T>* __restrict__ out, AsmArgs a, double L, size_t 
     const int d = a.comp % 3, dp = (d + 1) % 3, dpp = (d + 2) % 3;
     double g[3], vc = 1.0;
     if (a.plt.Dr) {
-        plt_grad(a.plt, a.n0, L, i, j, k, g, vel ? &vc : nullptr);
+        const double mr = (double)a.plt.ng / (double)a.n0;
+        plt_grad(a.plt, mr, 1.0 / (mr * L), i, j, k, g, vel ? &vc : nullptr);
     } else {
         g[0] = fgrad(i, a.n0, a.kf0);
         g[1] = fgrad(j, a.n1, a.kf1);
@@ -148,6 +166,7 @@ struct AsmAllArgs {
     double ulen, uvel;        // lunit / L, vunit / L
     double sh;                // sub-lattice shift (1/2) * dx
     PltDev plt;
+    double plt_mr, plt_c0;    // ng / n and n / (ng L) of the PLT lookup
 };
 
 // ORDER, PLT and NSUB are compile-time so that every field load of the mode
@@ -168,7 +187,7 @@ __global__ void k_assemble_all(AsmAllArgs a, size_t ntot) {
     const int j = (int)(r % a.n), i = (int)(r / a.n);
     double g[3], vc = 1.0;
     if constexpr (PLT) {
-        plt_grad(a.plt, a.n, a.L, i, j, k, g, &vc);
+        plt_grad(a.plt, a.plt_mr, a.plt_c0, i, j, k, g, &vc);
     } else {
         g[0] = fgrad(i, a.n, a.kf);
         g[1] = fgrad(j, a.n, a.kf);
@@ -309,6 +328,8 @@ void assemble_all(mfic_ctx ctx, int prec, size_t n, double L, int order, const d
     a.ulen = ulen; a.uvel = uvel;
     a.sh = 0.5 * L / (double)n;
     a.plt = plt;
+    a.plt_mr = (double)plt.ng / (double)n;
+    a.plt_c0 = plt.ng ? 1.0 / (a.plt_mr * L) : 0.0;
     const size_t ntot = n * n * (n / 2 + 1);
     ProfScope prof(ctx, plt.Dr ? "assemble_all_plt" : "assemble_all",
                    (double)ntot * (prec == MFIC_F64 ? 16 : 8) * (order + (order > 2 ? 3 : 0) + nout));
