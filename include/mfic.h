@@ -140,7 +140,9 @@ enum mfic_fft_pass {
     MFIC_PASS_PAD_R2C_A = 5,  /*   axis-0 r2c */
     MFIC_PASS_PAD_R2C_B = 6,  /*   axis-1 r2c */
     MFIC_PASS_TRUNC_AX0 = 7,  /*   two-stage truncation filter, axis 0 */
-    MFIC_PASS_TRUNC_AX1 = 8   /*   two-stage truncation filter, axis 1 */
+    MFIC_PASS_TRUNC_AX1 = 8,  /*   two-stage truncation filter, axis 1 */
+    MFIC_PASS_FUSED_ROWS = 9  /*   fused source row passes: variant 0 transforms two rows per
+                                   full-length complex FFT, 1 one row per half-length FFT */
 };
 /* number of compiled radix plans for a length (0: unsupported length) */
 int mfic_fft_plan_count(int length);

@@ -20,8 +20,10 @@ SC, BCC, FCC = 0, 1, 2
 # strided pass kinds with a selectable radix plan (mfic_fft_pass)
 PASS_PENCILS, PASS_AX0_SHIFT = 0, 2
 PASS_PAD_C2R_A, PASS_PAD_C2R_B, PASS_PAD_R2C_A, PASS_PAD_R2C_B, PASS_TRUNC_AX0, PASS_TRUNC_AX1 = 3, 4, 5, 6, 7, 8
+# fused source row passes: form 0 = two rows per full-length complex FFT, 1 = one half-length FFT per row
+PASS_FUSED_ROWS = 9
 PASSES = (PASS_PENCILS, PASS_AX0_SHIFT, PASS_PAD_C2R_A, PASS_PAD_C2R_B, PASS_PAD_R2C_A, PASS_PAD_R2C_B,
-          PASS_TRUNC_AX0, PASS_TRUNC_AX1)
+          PASS_TRUNC_AX0, PASS_TRUNC_AX1, PASS_FUSED_ROWS)
 
 ERRORS = {
     -1: "MFIC_ERR_ARG", -2: "MFIC_ERR_SIZE", -3: "MFIC_ERR_HIP", -4: "MFIC_ERR_ALLOC",

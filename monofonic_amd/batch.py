@@ -157,11 +157,15 @@ def chosen_plans(ctx: Context, n: int, prec: int) -> dict:
     size n can use"""
     out = {}
     for length in (n, 3 * n // 2):
-        if plan_count(length) > 1:
-            for p in L.PASSES:
-                v = get_plan(ctx, length, p, prec)
-                if v:
-                    out[(length, p)] = v
+        for p in L.PASSES:
+            if p == L.PASS_FUSED_ROWS:  # two forms at the padded length
+                if length != 3 * n // 2:
+                    continue
+            elif plan_count(length) < 2:
+                continue
+            v = get_plan(ctx, length, p, prec)
+            if v:
+                out[(length, p)] = v
     return out
 
 

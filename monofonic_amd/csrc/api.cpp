@@ -399,7 +399,7 @@ int mfic_fft_plan_info(int length, int variant, int* elems_per_thread, int radic
 static void check_plan_key(mfic_ctx ctx, int length, int pass, int prec) {
     MFIC_CHECK(ctx, MFIC_ERR_ARG, "null context");
     MFIC_CHECK(length > 0 && fft_length_supported(length), MFIC_ERR_SIZE, "unsupported FFT length");
-    MFIC_CHECK(pass == MFIC_PASS_PENCILS || (pass >= MFIC_PASS_AX0_SHIFT && pass <= MFIC_PASS_TRUNC_AX1), MFIC_ERR_ARG,
+    MFIC_CHECK(pass == MFIC_PASS_PENCILS || (pass >= MFIC_PASS_AX0_SHIFT && pass <= MFIC_PASS_FUSED_ROWS), MFIC_ERR_ARG,
                "pass must be one of mfic_fft_pass (row passes have one plan)");
     MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
 }
@@ -409,6 +409,11 @@ int mfic_fft_set_plan(mfic_ctx ctx, int length, int pass, int prec, int variant)
         check_plan_key(ctx, length, pass, prec);
         const auto key = std::make_tuple(length, pass, prec);
         if (variant <= 0) { ctx->plans.erase(key); return; }
+        if (pass == MFIC_PASS_FUSED_ROWS) {  // two forms at every padded length
+            MFIC_CHECK(variant < 2, MFIC_ERR_ARG, "the fused row passes have two forms");
+            ctx->plans[key] = variant;
+            return;
+        }
         MFIC_CHECK(variant < fft_plan_count(length), MFIC_ERR_ARG,
                    "length " + std::to_string(length) + " has " + std::to_string(fft_plan_count(length)) + " radix plan(s)");
         ctx->plans[key] = variant;

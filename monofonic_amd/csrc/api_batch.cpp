@@ -117,14 +117,18 @@ double autotune_config(mfic_ctx ctx, const mfic_ic_config& cfg, const mfic_cosmo
         };
         const bool shift = cfg.particles && cfg.lattice == MFIC_LATTICE_BCC;
         const int np = 3 * n / 2;
-        // plan knobs: (length, pass) with more than one compiled plan
-        std::vector<std::pair<int, int>> plan_knobs;
+        // plan knobs: (length, pass, variants) with more than one variant
+        struct PlanKnob { int length, pass, count; };
+        std::vector<PlanKnob> plan_knobs;
         if (mfic::fft_plan_count(n) > 1) {
-            plan_knobs.push_back({n, MFIC_PASS_PENCILS});
-            if (shift) plan_knobs.push_back({n, MFIC_PASS_AX0_SHIFT});
+            plan_knobs.push_back({n, MFIC_PASS_PENCILS, mfic::fft_plan_count(n)});
+            if (shift) plan_knobs.push_back({n, MFIC_PASS_AX0_SHIFT, mfic::fft_plan_count(n)});
         }
         if (cfg.lpt_order > 1 && mfic::fft_plan_count(np) > 1)
-            for (int pass = MFIC_PASS_PAD_C2R_A; pass <= MFIC_PASS_TRUNC_AX1; ++pass) plan_knobs.push_back({np, pass});
+            for (int pass = MFIC_PASS_PAD_C2R_A; pass <= MFIC_PASS_TRUNC_AX1; ++pass)
+                plan_knobs.push_back({np, pass, mfic::fft_plan_count(np)});
+        // the two forms of the fused source row passes
+        if (cfg.lpt_order > 1) plan_knobs.push_back({np, MFIC_PASS_FUSED_ROWS, 2});
         struct Knob { int length, kind; std::vector<int> cand; };
         std::vector<Knob> tile_knobs = {{n, 0, {4, 8, 16, 32, 64}}, {n / 2, 1, {1, 2, 4, 8, 16}}};
         if (shift) tile_knobs.push_back({n, 2, {4, 8, 16, 32}});
@@ -139,11 +143,11 @@ double autotune_config(mfic_ctx ctx, const mfic_ic_config& cfg, const mfic_cosmo
         float cur = time_once();
         auto tune_plans = [&] {
             for (const auto& k : plan_knobs) {
-                const auto key = std::make_tuple(k.first, k.second, prec);
-                const int had = mfic::plan_variant(ctx, k.first, k.second, prec);
+                const auto key = std::make_tuple(k.length, k.pass, prec);
+                const int had = mfic::plan_variant(ctx, k.length, k.pass, prec);
                 int best_v = had;
                 float best = cur;
-                for (int v = 0; v < mfic::fft_plan_count(k.first); ++v) {
+                for (int v = 0; v < k.count; ++v) {
                     if (v == had) continue;
                     if (v) ctx->plans[key] = v;
                     else ctx->plans.erase(key);

@@ -262,9 +262,10 @@ static std::vector<std::complex<long double>> make_table(int L, int kind, int va
     switch (L) {
 #define X(S)                                                     \
     case S:                                                      \
-        if (variant == 1) fill_tw<S, alt_plan<S>()>(tw);         \
-        else if (variant == 2) fill_tw<S, row_alt_plan<S>()>(tw); \
-        else fill_tw<S>(tw);                                     \
+        if (variant == 1) fill_tw<S, plan_or_default<S, 1>()>(tw);      \
+        else if (variant == 2) fill_tw<S, plan_or_default<S, 2>()>(tw); \
+        else if (variant == 3) fill_tw<S, plan_or_default<S, 3>()>(tw); \
+        else fill_tw<S>(tw);                                            \
         break;
         MFIC_FOR_SIZES(X)
 #undef X
@@ -286,8 +287,8 @@ static std::vector<std::complex<long double>> make_table(int L, int kind, int va
 
 const void* twiddles(mfic_ctx ctx, int L, int kind, int prec, int variant) {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    MFIC_CHECK(variant == 0 || (kind == 0 && variant < fft_plan_count(L)) || (kind == 1 && variant == 2 && L == 384),
-               MFIC_ERR_ARG, "no such radix plan");
+    MFIC_CHECK(variant >= 0 && variant <= 3 && kind >= 0 && kind <= 2 && plan_exists(L, variant), MFIC_ERR_ARG,
+               "no such radix plan");
     auto key = std::make_tuple(L, kind + 4 * variant, prec);
     auto it = ctx->tables.find(key);
     if (it != ctx->tables.end()) return it->second.dev;

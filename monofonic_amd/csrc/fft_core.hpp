@@ -174,6 +174,11 @@ MFIC_PLAN_V(768, 1, 48, 3, 16, 12, 4, 0)
 // row plan (V 2) for the output transforms of the 16-input fused pass: 32
 // threads per 384-point row
 MFIC_PLAN_V(384, 2, 12, 4, 12, 4, 4, 2)
+// row plans of the paired fused passes (two real rows per 768-point complex
+// transform): V 3 on 32 threads for the inputs, V 2 on 64 threads for the
+// outputs.  Radix orders whose exchanges have conflict-free swizzles.
+MFIC_PLAN_V(768, 2, 12, 4, 4, 12, 4, 4)
+MFIC_PLAN_V(768, 3, 24, 3, 8, 12, 8, 0)
 #undef MFIC_PLAN
 #undef MFIC_PLAN_V
 // number of radix plans compiled for a length
@@ -183,6 +188,16 @@ template <int L> constexpr int plan_variants() { return (L == 512 || L == 768) ?
 template <int L> constexpr int alt_plan() { return plan_variants<L>() > 1 ? 1 : 0; }
 // the row plan 2 of a length that has one, else plan 0
 template <int L> constexpr int row_alt_plan() { return L == 384 ? 2 : 0; }
+// whether plan V of length L is compiled (run-time form for table checks)
+__host__ __device__ constexpr bool plan_exists(int L, int V) {
+    return V == 0 || (V == 1 && (L == 512 || L == 768)) || (V == 2 && (L == 384 || L == 768)) || (V == 3 && L == 768);
+}
+// plan V of L if compiled, else plan 0 (compile-time dispatch helper)
+template <int L, int V> constexpr int plan_or_default() { return plan_exists(L, V) ? V : 0; }
+// plans of the paired fused passes: input transforms (one group of lanes
+// per pair of input rows) and output transforms (one per pair of outputs)
+template <int L> constexpr int pair_in_plan() { return L == 768 ? 3 : 0; }
+template <int L> constexpr int pair_out_plan() { return L == 768 ? 2 : 0; }
 
 template <int L, int V = 0> constexpr int plan_ns(int s) {  // span before stage s
     int ns = 1;
@@ -236,9 +251,10 @@ template <int M> constexpr int row_lds_pitch() {
 // stage hits distinct 8-byte bank slots within each 16- and 32-lane group
 // (exhaustive search over a, b, k1, k2 against the access pattern of the
 // plan).  kind 0: identity; kind 2: rotation instead of xor.
-template <int L, int S> struct RowSwz { static constexpr int kind = 0, a = 0, k1 = 0, b = 0, k2 = 0; };
-#define MFIC_SWZ(L_, S_, KIND, A, K1, B, K2) \
-    template <> struct RowSwz<L_, S_> { static constexpr int kind = KIND, a = A, k1 = K1, b = B, k2 = K2; };
+template <int L, int S, int V = 0> struct RowSwz { static constexpr int kind = 0, a = 0, k1 = 0, b = 0, k2 = 0; };
+#define MFIC_SWZ_V(L_, S_, V_, KIND, A, K1, B, K2) \
+    template <> struct RowSwz<L_, S_, V_> { static constexpr int kind = KIND, a = A, k1 = K1, b = B, k2 = K2; };
+#define MFIC_SWZ(L_, S_, KIND, A, K1, B, K2) MFIC_SWZ_V(L_, S_, 0, KIND, A, K1, B, K2)
 MFIC_SWZ(24, 0, 1, 1, 1, 1, 0)
 MFIC_SWZ(32, 0, 1, 2, 1, 2, 0)
 MFIC_SWZ(36, 0, 1, 1, 0, 3, 1)
@@ -258,11 +274,18 @@ MFIC_SWZ(384, 0, 1, 3, 0, 4, 1)
 MFIC_SWZ(384, 1, 1, 4, 0, 6, 3)
 MFIC_SWZ(512, 0, 1, 4, 1, 5, 1)
 MFIC_SWZ(512, 1, 1, 5, 0, 6, 8)
+// paired fused-pass plans (one transform per 32 or 64 lanes)
+MFIC_SWZ_V(768, 0, 3, 1, 3, 0, 5, 1)
+MFIC_SWZ_V(768, 1, 3, 1, 5, 0, 5, 8)
+MFIC_SWZ_V(768, 0, 2, 1, 2, 0, 5, 1)
+MFIC_SWZ_V(768, 1, 2, 1, 4, 0, 4, 2)
+MFIC_SWZ_V(768, 2, 2, 1, 5, 0, 5, 4)
 #undef MFIC_SWZ
-template <int L, int S> struct LdsSwzMap {
+#undef MFIC_SWZ_V
+template <int L, int S, int V = 0> struct LdsSwzMap {
     int base;
     __device__ __forceinline__ int at(int x) const {
-        using W = RowSwz<L, S>;
+        using W = RowSwz<L, S, V>;
         static_assert(W::kind == 0 || L % (1 << W::a) == 0, "swizzle must stay inside the row");
         constexpr int A = (1 << W::a) - 1;
         if constexpr (W::kind == 0) return base + x;
@@ -381,14 +404,14 @@ __device__ __forceinline__ void fft_pencil(cx<T>* v, int q, cx<T>* lds, LdsMap m
 template <typename T, int L, bool FWD, class SYNC = BlockSync, int V = 0>
 __device__ __forceinline__ void fft_regs_row(cx<T>* v, int q, cx<T>* lds, int base, const cx<T>* __restrict__ tw) {
     constexpr int S = Plan<L, V>::S;
-    if constexpr (V != 0) {  // row plans other than the default: plain row addressing
-        fft_regs<T, L, FWD, LdsMap, SYNC, false, V>(v, q, lds, LdsMap{base, 1}, tw);
-    } else {
-        fft_stage<T, L, 0, FWD, LdsSwzMap<L, 0>, SYNC>(v, q, lds, LdsSwzMap<L, 0>{base}, tw);
-        if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, LdsSwzMap<L, 1 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 1 % S>{base}, tw);
-        if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, LdsSwzMap<L, 2 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 2 % S>{base}, tw);
-        if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, LdsSwzMap<L, 3 % S>, SYNC>(v, q, lds, LdsSwzMap<L, 3 % S>{base}, tw);
-    }
+    using M0 = LdsSwzMap<L, 0, V>;
+    using M1 = LdsSwzMap<L, 1 % S, V>;
+    using M2 = LdsSwzMap<L, 2 % S, V>;
+    using M3 = LdsSwzMap<L, 3 % S, V>;
+    fft_stage<T, L, 0, FWD, M0, SYNC, false, V>(v, q, lds, M0{base}, tw);
+    if constexpr (S > 1) fft_stage<T, L, 1 % S, FWD, M1, SYNC, false, V>(v, q, lds, M1{base}, tw);
+    if constexpr (S > 2) fft_stage<T, L, 2 % S, FWD, M2, SYNC, false, V>(v, q, lds, M2{base}, tw);
+    if constexpr (S > 3) fft_stage<T, L, 3 % S, FWD, M3, SYNC, false, V>(v, q, lds, M3{base}, tw);
 }
 
 }  // namespace mfic

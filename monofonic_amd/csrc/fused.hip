@@ -228,6 +228,179 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
     }
 }
 
+// Paired form of the fused row pass: two real rows per complex transform of
+// the full padded length L2 = 2M.  Inputs 2g and 2g+1 of a row are brought
+// to real space together, z = x_2g + i x_2g+1, from the Hermitian spectrum
+// Z[k] = X_2g[k] + i X_2g+1[k] (k <= N/2) and conj(X_2g[L2-k]) +
+// i conj(X_2g+1[L2-k]) (k >= L2 - N/2), read straight from the compact rows:
+// no half-length untangle, no LDS round trip before the transform.  Outputs
+// are paired the same way, w = o_2p + i o_2p+1, and separated after the
+// forward transform as O_2p[k] = W[k] + conj(W[L2-k]),
+// O_2p+1[k] = -i (W[k] - conj(W[L2-k])) (the 1/2 and both normalisations
+// are one FP64 factor on the products).
+template <int L2, int GI> constexpr int pair_rows_per_block() {
+    constexpr int tpr = GI * lane_slot<Plan<L2, pair_in_plan<L2>()>::P>();
+    return tpr >= 256 ? 1 : (256 / tpr > 8 ? 8 : 256 / tpr);
+}
+template <int L2, class PR> constexpr int pair_block_threads() {
+    constexpr int GI = PR::NI / 2;
+    return pair_rows_per_block<L2, GI>() * GI * lane_slot<Plan<L2, pair_in_plan<L2>()>::P>();
+}
+// output transforms on 64 lanes (plan pair_out_plan) where every output
+// pair of the block gets a wavefront, else on the input transforms' lanes
+template <int L2, class PR> constexpr int pair_out_variant() {
+    constexpr int GO = (PR::NO + 1) / 2, RB = pair_rows_per_block<L2, PR::NI / 2>();
+    constexpr int VO = pair_out_plan<L2>();
+    return VO != 0 && RB * GO * lane_slot<Plan<L2, VO>::P>() <= pair_block_threads<L2, PR>() ? VO
+                                                                                            : pair_in_plan<L2>();
+}
+
+template <typename T, int L2, class PR>
+__global__ void __launch_bounds__((pair_block_threads<L2, PR>()))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PR::WAVES : 1, 8)))
+k_rows_fused_pair(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out, long ostride, long nrows,
+                  double scale2, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ two) {
+    constexpr int NI = PR::NI, NO = PR::NO, GI = NI / 2, GO = (NO + 1) / 2;
+    constexpr int VI = pair_in_plan<L2>(), VO = pair_out_variant<L2, PR>();
+    constexpr int E = Plan<L2, VI>::E, P = Plan<L2, VI>::P, SL = lane_slot<P>();
+    constexpr int EO = Plan<L2, VO>::E, PO = Plan<L2, VO>::P, SLO = lane_slot<PO>();
+    constexpr int RB = pair_rows_per_block<L2, GI>(), TPR = GI * SL, RP = L2;
+    constexpr int KC = L2 / 3 + 1;  // N/2 + 1 values per compact row, N = 2 L2 / 3
+    constexpr int NPOS = (L2 + TPR - 1) / TPR;
+    static_assert(NI % 2 == 0, "inputs come in pairs");
+    static_assert(P <= 64 && PO <= 64 && 64 % SL == 0 && 64 % SLO == 0, "a transform must fit in one wavefront");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int tr = threadIdx.x % TPR, rb = threadIdx.x / TPR;
+    const int g = tr / SL, q = tr % SL;
+    const long tile = xcd_tile(blockIdx.x, gridDim.x);
+    const long row = tile * RB + rb;
+    const bool active = row < nrows;
+
+    // 1. inverse transform of input pair g: all compact-row loads first, the
+    // zero band (N/2 < k < L2 - N/2) never loaded
+    if (q < P) {
+        const long r0 = (active ? row : 0) * (long)KC;
+        const cx<T>* s0 = in + (2 * g) * istride + r0;
+        const cx<T>* s1 = in + (2 * g + 1) * istride + r0;
+        cx<T> a[E], b[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            a[e] = b[e] = cx<T>{0, 0};
+            if (P * e + P <= KC) {
+                a[e] = s0[k];
+                b[e] = s1[k];
+            } else if (P * e > L2 - KC) {
+                a[e] = s0[L2 - k];
+                b[e] = s1[L2 - k];
+            } else if (P * e < KC || P * e + P > L2 - KC + 1) {
+                if (k < KC) {
+                    a[e] = s0[k];
+                    b[e] = s1[k];
+                } else if (k > L2 - KC) {
+                    a[e] = s0[L2 - k];
+                    b[e] = s1[L2 - k];
+                }
+            }
+        }
+        cx<T> v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k = q + P * e;
+            if (P * e > L2 - KC || (P * e + P > L2 - KC + 1 && k > L2 - KC)) {
+                v[e] = cx<T>{a[e].x + b[e].y, b[e].x - a[e].y};  // conj(A) + i conj(B)
+            } else {
+                if (e == 0 && k == 0) a[e].y = b[e].y = 0;  // the DC imaginary parts are not data
+                v[e] = cx<T>{a[e].x - b[e].y, a[e].y + b[e].x};  // A + i B
+            }
+        }
+        const int base = (rb * GI + g) * RP;
+        fft_regs_row<T, L2, false, WaveSync, VI>(v, q, lds, base, tw);
+#pragma unroll
+        for (int e = 0; e < E; ++e) lds[base + q + P * e] = v[e];
+    }
+    __syncthreads();
+
+    // 2. products at every real point n from the NI/2 pair rows, in FP64,
+    // times the normalisation; outputs paired over the first pair rows (a
+    // point is read and written by one thread only)
+#pragma unroll
+    for (int u = 0; u < NPOS; ++u) {
+        const int n = tr + u * TPR;
+        if (n < L2) {
+            double h[NI], o[NO];
+#pragma unroll
+            for (int p = 0; p < GI; ++p) {
+                const cx<T> z = lds[(rb * GI + p) * RP + n];
+                h[2 * p] = (double)z.x;
+                h[2 * p + 1] = (double)z.y;
+            }
+            PR::eval(h, o);
+#pragma unroll
+            for (int p = 0; p < GO; ++p) {
+                const T lo = (T)(o[2 * p] * scale2);
+                const T hi = 2 * p + 1 < NO ? (T)(o[2 * p + 1] * scale2) : (T)0;
+                lds[(rb * GI + p) * RP + n] = cx<T>{lo, hi};
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. forward transform of output pair p, separated into the two outputs'
+    // first KC modes
+    {
+        const int t = threadIdx.x / SLO, qo = threadIdx.x % SLO;
+        const int rbo = t / GO, p = t % GO;
+        if (t < RB * GO && qo < PO) {
+            const int base = (rbo * GI + p) * RP;
+            cx<T> v[EO];
+#pragma unroll
+            for (int e = 0; e < EO; ++e) v[e] = lds[base + qo + PO * e];
+            WaveSync::sync();
+            fft_regs_row<T, L2, true, WaveSync, VO>(v, qo, lds, base, two);
+            // partners W[L2 - k] of the kept modes sit at L2 - KC + 1 .. L2 - 1
+#pragma unroll
+            for (int e = 0; e < EO; ++e)
+                if (PO * e + PO > L2 - KC + 1) lds[base + qo + PO * e] = v[e];
+            WaveSync::sync();
+            const long rowo = tile * RB + rbo;
+            if (rowo < nrows) {
+                cx<T>* d0 = out + (2 * p) * ostride + rowo * (long)KC;
+                cx<T>* d1 = out + (2 * p + 1 < NO ? 2 * p + 1 : 2 * p) * ostride + rowo * (long)KC;
+#pragma unroll
+                for (int e = 0; e < EO; ++e) {
+                    const int k = qo + PO * e;
+                    if (PO * e >= KC || k >= KC) continue;
+                    const cx<T> w = v[e], wm = k == 0 ? v[e] : lds[base + L2 - k];
+                    d0[k] = cx<T>{w.x + wm.x, w.y - wm.y};
+                    if (2 * p + 1 < NO) d1[k] = cx<T>{w.y + wm.y, wm.x - w.x};
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int L2, class PR>
+static void launch_fused_pair(mfic_ctx ctx, const void* in, void* out, long stride, int kc, long nrows, double scale,
+                              const char* name) {
+    constexpr int NI = PR::NI, NO = PR::NO, GI = NI / 2;
+    constexpr int RB = pair_rows_per_block<L2, GI>(), VI = pair_in_plan<L2>(), VO = pair_out_variant<L2, PR>();
+    MFIC_CHECK(3 * (kc - 1) == L2, MFIC_ERR_SIZE, "half-spectrum rows must hold N/2 + 1 values of a 3N/2 padded row");
+    const size_t lds = (size_t)RB * GI * L2 * sizeof(cx<T>);
+    MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
+    const int prec = sizeof(T) == 8 ? MFIC_F64 : MFIC_F32;
+    const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L2, 0, prec, VI));
+    const cx<T>* two = static_cast<const cx<T>*>(twiddles(ctx, L2, 0, prec, VO));
+    dim3 grid((unsigned)xcd_grid((nrows + RB - 1) / RB));
+    ProfScope prof(ctx, std::string(name) + "_n" + std::to_string(L2) + (sizeof(T) == 8 ? "_f64" : "_f32"),
+                   (double)nrows * (NI + NO) * kc * sizeof(cx<T>));
+    hipLaunchKernelGGL((k_rows_fused_pair<T, L2, PR>), grid, dim3(pair_block_threads<L2, PR>()), lds, ctx->stream,
+                       static_cast<const cx<T>*>(in), stride, static_cast<cx<T>*>(out), stride, nrows,
+                       0.5 * scale * scale, tw, two);
+    MFIC_LAUNCH_CHECK();
+}
+
 template <typename T, int M, class PR>
 static void launch_fused(mfic_ctx ctx, const void* in, void* out, long stride, int kc, long nrows, double scale,
                          const char* name) {
@@ -357,10 +530,14 @@ static void fused_t(mfic_ctx ctx, size_t n, const void* in, void* out, const cha
     const int N = (int)n, np = 3 * N / 2, kc = N / 2 + 1;
     const long stride = (long)np * np * kc;
     const double norm = 1.0 / std::sqrt((double)np * (double)np * (double)np);
+    // form 0 (default): two rows per full-length transform; form 1: one
+    // half-length transform per row
+    const bool paired = plan_variant(ctx, np, MFIC_PASS_FUSED_ROWS, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32) == 0;
     switch (np) {
 #define X(S)                                                                              \
     case S:                                                                               \
-        launch_fused<T, S / 2, PR>(ctx, in, out, stride, kc, (long)np * np, norm, name); \
+        if (paired) launch_fused_pair<T, S, PR>(ctx, in, out, stride, kc, (long)np * np, norm, name); \
+        else launch_fused<T, S / 2, PR>(ctx, in, out, stride, kc, (long)np * np, norm, name); \
         break;
         MFIC_FUSED_SIZES(X)
 #undef X

@@ -122,8 +122,10 @@ def test_c2_potentials_and_orders_match_oracle(ctx, prec):
 # -------------------------------------------------------------------- C3 --
 def _set_plans(ctx, n, prec, variant):
     """every strided pass of the n^3 run (N-grid pencils, shifted
-    sub-lattice pass, padded convolver passes) on radix plan `variant`
-    (-1: back to the defaults)"""
+    sub-lattice pass, padded convolver passes) on radix plan `variant`, and
+    the fused source row passes in form `variant` (1: one half-length
+    transform per row instead of two rows per full-length transform; -1:
+    back to the defaults)"""
     from monofonic_amd import batch
     for length in (n, 3 * n // 2):
         if batch.plan_count(length) > 1:
