@@ -176,9 +176,11 @@ MFIC_PLAN_V(768, 1, 48, 3, 16, 12, 4, 0)
 MFIC_PLAN_V(384, 2, 12, 4, 12, 4, 4, 2)
 // row plans of the paired fused passes (two real rows per 768-point complex
 // transform): V 3 on 32 threads for the inputs, V 2 on 64 threads for the
-// outputs.  Radix orders whose exchanges have conflict-free swizzles.
-MFIC_PLAN_V(768, 2, 12, 4, 4, 12, 4, 4)
-MFIC_PLAN_V(768, 3, 24, 3, 8, 12, 8, 0)
+// outputs.  Radix orders with power-of-two spans before the last stage
+// (shift-and-mask twiddle indices) whose exchanges have conflict-free
+// swizzles.
+MFIC_PLAN_V(768, 2, 12, 4, 4, 4, 4, 12)
+MFIC_PLAN_V(768, 3, 24, 3, 8, 8, 12, 0)
 #undef MFIC_PLAN
 #undef MFIC_PLAN_V
 // number of radix plans compiled for a length
@@ -274,12 +276,16 @@ MFIC_SWZ(384, 0, 1, 3, 0, 4, 1)
 MFIC_SWZ(384, 1, 1, 4, 0, 6, 3)
 MFIC_SWZ(512, 0, 1, 4, 1, 5, 1)
 MFIC_SWZ(512, 1, 1, 5, 0, 6, 8)
-// paired fused-pass plans (one transform per 32 or 64 lanes)
-MFIC_SWZ_V(768, 0, 3, 1, 3, 0, 5, 1)
-MFIC_SWZ_V(768, 1, 3, 1, 5, 0, 5, 8)
-MFIC_SWZ_V(768, 0, 2, 1, 2, 0, 5, 1)
-MFIC_SWZ_V(768, 1, 2, 1, 4, 0, 4, 2)
-MFIC_SWZ_V(768, 2, 2, 1, 5, 0, 5, 4)
+// paired fused-pass plans (one transform per 32 or 64 lanes): searched
+// against 64-bit LDS accesses served 16 lanes at a time by 16 8-byte slots
+// (equivalently a wavefront by 64 slots), the model that reproduces the
+// SQ_LDS_BANK_CONFLICT counts measured on MI355X, with two 32-lane
+// transforms per wavefront sitting 800 slots apart
+MFIC_SWZ_V(768, 0, 3, 1, 3, 2, 5, 1)
+MFIC_SWZ_V(768, 1, 3, 1, 5, 0, 5, 4)
+MFIC_SWZ_V(768, 0, 2, 1, 4, 1, 6, 1)
+MFIC_SWZ_V(768, 1, 2, 1, 4, 0, 4, 5)
+MFIC_SWZ_V(768, 2, 2, 1, 6, 16, 6, 0)
 #undef MFIC_SWZ
 #undef MFIC_SWZ_V
 template <int L, int S, int V = 0> struct LdsSwzMap {

@@ -255,6 +255,11 @@ template <int L2, class PR> constexpr int pair_out_variant() {
                                                                                             : pair_in_plan<L2>();
 }
 
+// LDS slots per pair row: the first pitch of at least L2 that is 32 modulo
+// 64, so the two rows a wavefront covers (two 32-lane groups) fall on
+// opposite halves of the 64 bank slots
+template <int L2> constexpr int pair_row_pitch() { return L2 + ((32 - L2 % 64) % 64 + 64) % 64; }
+
 template <typename T, int L2, class PR>
 __global__ void __launch_bounds__((pair_block_threads<L2, PR>()))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PR::WAVES : 1, 8)))
@@ -264,7 +269,7 @@ k_rows_fused_pair(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict_
     constexpr int VI = pair_in_plan<L2>(), VO = pair_out_variant<L2, PR>();
     constexpr int E = Plan<L2, VI>::E, P = Plan<L2, VI>::P, SL = lane_slot<P>();
     constexpr int EO = Plan<L2, VO>::E, PO = Plan<L2, VO>::P, SLO = lane_slot<PO>();
-    constexpr int RB = pair_rows_per_block<L2, GI>(), TPR = GI * SL, RP = L2;
+    constexpr int RB = pair_rows_per_block<L2, GI>(), TPR = GI * SL, RP = pair_row_pitch<L2>();
     constexpr int KC = L2 / 3 + 1;  // N/2 + 1 values per compact row, N = 2 L2 / 3
     constexpr int NPOS = (L2 + TPR - 1) / TPR;
     static_assert(NI % 2 == 0, "inputs come in pairs");
@@ -387,7 +392,7 @@ static void launch_fused_pair(mfic_ctx ctx, const void* in, void* out, long stri
     constexpr int NI = PR::NI, NO = PR::NO, GI = NI / 2;
     constexpr int RB = pair_rows_per_block<L2, GI>(), VI = pair_in_plan<L2>(), VO = pair_out_variant<L2, PR>();
     MFIC_CHECK(3 * (kc - 1) == L2, MFIC_ERR_SIZE, "half-spectrum rows must hold N/2 + 1 values of a 3N/2 padded row");
-    const size_t lds = (size_t)RB * GI * L2 * sizeof(cx<T>);
+    const size_t lds = (size_t)RB * GI * pair_row_pitch<L2>() * sizeof(cx<T>);
     MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
     const int prec = sizeof(T) == 8 ? MFIC_F64 : MFIC_F32;
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L2, 0, prec, VI));
