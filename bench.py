@@ -522,12 +522,13 @@ def main():
             set_plan(ctx, length, pass_kind, F32, variant)
         args.no_tune = True
     if not args.no_tune:
-        from monofonic_amd.batch import autotune_config, chosen_plans
+        from monofonic_amd.batch import autotune_config, chosen_plans, chosen_tiles
         from monofonic_amd.lpt import baseline_cosmology
         log(f"autotune radix plans and tiles for the {n}^3 3LPT+PLT bcc FP32 run (plan time)")
         tuned_ms = autotune_config(ctx, cfg, baseline_cosmology(24.0))
         plans = {f"L{k[0]}_pass{k[1]}": v for k, v in chosen_plans(ctx, n, F32).items()}
-        log(f"tuned: {tuned_ms:.2f} ms per execution, non-default plans {plans}")
+        tiles = {f"L{k[0]}_kind{k[1]}": v for k, v in chosen_tiles(ctx, n, F32).items()}
+        log(f"tuned: {tuned_ms:.2f} ms per execution, non-default plans {plans}, tiles {tiles}")
     log(f"plan: 3LPT+PLT bcc n={n} box={box} FP32")
     run = ICRun(ctx, cfg)
     log(f"timing {args.steps} steps after {args.warmup} warmup")
@@ -542,7 +543,7 @@ def main():
         "config": {"workload": f"3LPT+PLT bcc {n}^3 ({3 * n // 2}^3 padded), box {box:g} Mpc/h, z_start 24, "
                                "FP32 bulk, FP64 q+Psi", "n": n, "padded": 3 * n // 2, "box": box, "zstart": 24.0,
                    "lpt_order": 3, "lattice": "bcc", "plt": True, "particles": 2 * n ** 3},
-        "autotune": None if args.no_tune else {"tuned_ms": round(tuned_ms, 3), "plans": plans},
+        "autotune": None if args.no_tune else {"tuned_ms": round(tuned_ms, 3), "plans": plans, "tiles": tiles},
         "roofline": roofline(rep, args.steps),
         "kernels": kernel_table(rep, args.steps),
     }

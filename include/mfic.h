@@ -125,6 +125,8 @@ int mfic_fft_size_supported(size_t n);
  * restores the built-in choice.  No reference counterpart (FFTW planning,
  * grid_fft.cc:53-54, plays this role there). */
 int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, int lds_pad);
+/* the override of mfic_fft_set_tile for a key (tile 0: the built-in choice) */
+int mfic_fft_get_tile(mfic_ctx ctx, int length, int kind, int prec, int *tile, int *lds_pad);
 
 /* Radix plans.  Each transform length has a compiled default plan (elements
  * per thread and the radix of each Stockham stage); the padded-pass lengths

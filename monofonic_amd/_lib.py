@@ -110,6 +110,7 @@ SIGNATURES = {
     "mfic_fft_backward": (C.c_int, [_P]),
     "mfic_fft_size_supported": (C.c_int, [C.c_size_t]),
     "mfic_fft_set_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "mfic_fft_get_tile": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mfic_fft_plan_count": (C.c_int, [C.c_int]),
     "mfic_fft_plan_info": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mfic_fft_set_plan": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),

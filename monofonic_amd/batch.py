@@ -169,6 +169,23 @@ def chosen_plans(ctx: Context, n: int, prec: int) -> dict:
     return out
 
 
+def chosen_tiles(ctx: Context, n: int, prec: int) -> dict:
+    """{(length, kind): tile} of every tile override (autotuned or set) a
+    box of size n can use: kinds 0-2 at n and n/2, the padded pass kinds at
+    3n/2"""
+    out = {}
+    keys = [(n, 0), (n // 2, 1), (n, 2), (3 * n // 2, 0), (3 * n // 4, 1)]
+    keys += [(3 * n // 2, k) for k in range(L.PASS_PAD_C2R_A, L.PASS_TRUNC_AX1 + 1)]
+    for length, kind in keys:
+        if not L.load().mfic_fft_size_supported(length):
+            continue
+        t, pad = C.c_int(), C.c_int()
+        L.call("mfic_fft_get_tile", ctx.h, length, kind, prec, C.byref(t), C.byref(pad))
+        if t.value:
+            out[(length, kind)] = t.value
+    return out
+
+
 class Batch:
     """A list of boxes run back to back on one context, optionally as one graph."""
 

@@ -386,6 +386,18 @@ int mfic_fft_set_tile(mfic_ctx ctx, int length, int kind, int prec, int tile, in
     });
 }
 
+int mfic_fft_get_tile(mfic_ctx ctx, int length, int kind, int prec, int* tile, int* lds_pad) {
+    return guard([&] {
+        MFIC_CHECK(ctx && tile && lds_pad, MFIC_ERR_ARG, "null argument");
+        MFIC_CHECK(fft_length_supported(length), MFIC_ERR_SIZE, "unsupported FFT length");
+        MFIC_CHECK(kind >= 0 && kind <= MFIC_PASS_TRUNC_AX1, MFIC_ERR_ARG, "bad tile kind");
+        MFIC_CHECK(prec == MFIC_F32 || prec == MFIC_F64, MFIC_ERR_ARG, "bad precision");
+        const auto it = ctx->tiles.find(std::make_tuple(length, kind, prec));
+        *tile = it == ctx->tiles.end() ? 0 : it->second.first;
+        *lds_pad = it == ctx->tiles.end() ? 0 : it->second.second;
+    });
+}
+
 int mfic_fft_plan_count(int length) { return length > 0 ? fft_plan_count(length) : 0; }
 
 int mfic_fft_plan_info(int length, int variant, int* elems_per_thread, int radices[4]) {

@@ -57,13 +57,11 @@ __device__ inline double cic_apply(const double* __restrict__ T, const CicStenci
     return v;
 }
 
-// lattice_gradient::gradient for all three axes + vfac_corr
-// (particle_plt.hh:551-585), for N-grid mode (i, j, k); mr = ng / n and
-// c0 = 1 / (mr L) come from the host
-__device__ inline void plt_grad(const PltDev& P, double mr, double c0, int i, int j, int k, double g[3],
-                                double* vcorr) {
-    const int ng = P.ng;
-    const double ix = i * mr, iy = j * mr, iz = k * mr;
+// lattice_gradient::gradient for all three axes (particle_plt.hh:551-585)
+// from the interpolated eigenvalue tables Dr, Dtheta, Dphi at the mode's
+// table position (ix, iy, iz) = (i, j, k) * ng / n; c0 = 1 / (mr L)
+__device__ inline void plt_combine(int ng, double c0, double ix, double iy, double iz, double Dr, double Dt,
+                                   double Dp, double g[3]) {
     const double kf = 2.0 * 3.14159265358979323846;
     const double hn = (double)(ng / 2);
     const double kx = (ix - (double)(ix > hn) * ng) * kf, ky = (iy - (double)(iy > hn) * ng) * kf,
@@ -71,9 +69,6 @@ __device__ inline void plt_grad(const PltDev& P, double mr, double c0, int i, in
     const double kp2 = kx * kx + ky * ky;
     const double kr = sqrt(kp2 + kz * kz);
     const double kmod = kr * c0;
-    CicStencil cs;
-    cic_stencil(ng, ix, iy, iz, cs);
-    const double Dr = cic_apply(P.Dr, cs), Dt = cic_apply(P.Dtheta, cs), Dp = cic_apply(P.Dphi, cs);
     // spherical basis of k without trigonometry: theta = acos(kz/|k|),
     // phi = atan2(ky, kx) (phi = 0 on the kz axis, as atan2(0, 0))
     double st = 0.0, ct = 1.0, sp = 0.0, cp = 1.0;
@@ -90,7 +85,51 @@ __device__ inline void plt_grad(const PltDev& P, double mr, double c0, int i, in
     g[0] = kmod * (Dr * st * cp + Dt * ct * cp - Dp * sp);
     g[1] = kmod * (Dr * st * sp + Dt * ct * sp + Dp * cp);
     g[2] = kmod * (Dr * ct - Dt * st);
+}
+
+// lattice_gradient::gradient + vfac_corr (particle_plt.hh:551-585) for
+// N-grid mode (i, j, k) with the eight-corner CIC lookups of the tables;
+// mr = ng / n and c0 = 1 / (mr L) come from the host
+__device__ inline void plt_grad(const PltDev& P, double mr, double c0, int i, int j, int k, double g[3],
+                                double* vcorr) {
+    const double ix = i * mr, iy = j * mr, iz = k * mr;
+    CicStencil cs;
+    cic_stencil(P.ng, ix, iy, iz, cs);
+    plt_combine(P.ng, c0, ix, iy, iz, cic_apply(P.Dr, cs), cic_apply(P.Dtheta, cs), cic_apply(P.Dphi, cs), g);
     if (vcorr) *vcorr = cic_apply(P.vfac, cs);
+}
+
+// The CIC weights of a lookup factor into x-y (fixed along a row (i, j) of
+// modes) and z.  A block stages, for each of its rows, the x-y bilinear sums
+// of the four tables at every table z (ng/2 + 1 values per table) in LDS;
+// each mode then interpolates in z: two LDS reads per table instead of
+// eight table gathers.  Rows row0 .. row0 + nrow - 1 (rows past n^2 unused).
+__device__ inline void plt_stage_rows(const PltDev& P, double mr, int n, size_t row0, int nrow, double* sxy) {
+    const int ng = P.ng, ncg = ng / 2 + 1;
+    for (int v = threadIdx.x; v < nrow * 4 * ncg; v += blockDim.x) {
+        const int z = v % ncg, tab = (v / ncg) % 4, rr = v / (4 * ncg);
+        const size_t row = row0 + rr;
+        if (row >= (size_t)n * n) continue;
+        const int i = (int)(row / n), j = (int)(row % n);
+        const double x = i * mr, y = j * mr;
+        const int ix = (int)floor(x), iy = (int)floor(y);
+        const double dx = x - ix, tx = 1.0 - dx, dy = y - iy, ty = 1.0 - dy;
+        const int ix1 = (ix + 1) % ng, iy1 = (iy + 1) % ng;
+        const double* T = tab == 0 ? P.Dr : tab == 1 ? P.Dtheta : tab == 2 ? P.Dphi : P.vfac;
+        auto at = [&](int a, int b) { return T[((size_t)a * ng + b) * ncg + z]; };
+        sxy[v] = at(ix, iy) * (tx * ty) + at(ix, iy1) * (tx * dy) + at(ix1, iy) * (dx * ty) + at(ix1, iy1) * (dx * dy);
+    }
+}
+// plt_grad from the staged rows: sxy points at the mode's row
+__device__ inline void plt_grad_staged(const double* __restrict__ sxy, int ng, double mr, double c0, int i, int j,
+                                       int k, double g[3], double* vcorr) {
+    const int ncg = ng / 2 + 1;
+    const double iz = k * mr;
+    const int z0 = (int)floor(iz), z1 = min(z0 + 1, ncg - 1);
+    const double dz = iz - z0, tz = 1.0 - dz;
+    auto lerp = [&](int tab) { return sxy[tab * ncg + z0] * tz + sxy[tab * ncg + z1] * dz; };
+    plt_combine(ng, c0, i * mr, j * mr, iz, lerp(0), lerp(1), lerp(2), g);
+    *vcorr = lerp(3);
 }
 
 template <typename T>
@@ -170,10 +209,21 @@ struct AsmAllArgs {
 };
 
 // ORDER, PLT and NSUB are compile-time so that every field load of the mode
-// is issued before the gradient (a CIC gather for PLT) is evaluated
-template <typename T, int ORDER, bool PLT, int NSUB>
-__global__ void k_assemble_all(AsmAllArgs a, size_t ntot) {
+// is issued before the gradient (a CIC gather for PLT) is evaluated.  STAGE:
+// PLT lookups from per-row x-y sums in LDS (plt_stage_rows), blocks of
+// ASM_BLOCK modes spanning at most asm_stage_rows(nc) rows.
+constexpr int ASM_BLOCK = 256;
+__host__ __device__ constexpr int asm_stage_rows(int nc) { return (ASM_BLOCK + nc - 1) / nc + 1; }
+template <typename T, int ORDER, bool PLT, int NSUB, bool STAGE = false>
+__global__ void __launch_bounds__(ASM_BLOCK) k_assemble_all(AsmAllArgs a, size_t ntot) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* sxy = reinterpret_cast<double*>(smem_raw);
+    const size_t row0 = (size_t)blockIdx.x * blockDim.x / a.nc;
+    if constexpr (STAGE) {
+        plt_stage_rows(a.plt, a.plt_mr, a.n, row0, asm_stage_rows(a.nc), sxy);
+        __syncthreads();
+    }
     if (idx >= ntot) return;
     cx<T> pv[ORDER], av[3];
 #pragma unroll
@@ -186,7 +236,9 @@ __global__ void k_assemble_all(AsmAllArgs a, size_t ntot) {
     const size_t r = idx / a.nc;
     const int j = (int)(r % a.n), i = (int)(r / a.n);
     double g[3], vc = 1.0;
-    if constexpr (PLT) {
+    if constexpr (STAGE) {
+        plt_grad_staged(sxy + (r - row0) * 4 * (a.plt.ng / 2 + 1), a.plt.ng, a.plt_mr, a.plt_c0, i, j, k, g, &vc);
+    } else if constexpr (PLT) {
         plt_grad(a.plt, a.plt_mr, a.plt_c0, i, j, k, g, &vc);
     } else {
         g[0] = fgrad(i, a.n, a.kf);
@@ -335,15 +387,23 @@ void assemble_all(mfic_ctx ctx, int prec, size_t n, double L, int order, const d
                    (double)ntot * (prec == MFIC_F64 ? 16 : 8) * (order + (order > 2 ? 3 : 0) + nout));
     MFIC_CHECK(order >= 1 && order <= 3 && (nsub == 1 || nsub == 2), MFIC_ERR_ARG, "assembly: bad order or sub-lattices");
     const int key = (order - 1) * 4 + (plt.Dr ? 2 : 0) + (nsub - 1);
+    // staged PLT lookups where a block's rows are few (long rows)
+    const size_t stage_lds = (size_t)asm_stage_rows(a.nc) * 4 * (plt.ng / 2 + 1) * sizeof(double);
+    const bool stage = plt.Dr && a.nc >= 128 && stage_lds <= 32768;
+    const dim3 grid(nblk(ntot, ASM_BLOCK)), block(ASM_BLOCK);
     switch (key) {
-#define ASM(O, P, S)                                                                                             \
-    case (O - 1) * 4 + (P ? 2 : 0) + (S - 1):                                                                    \
-        if (prec == MFIC_F64)                                                                                    \
-            hipLaunchKernelGGL((k_assemble_all<double, O, P, S>), dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, \
-                               a, ntot);                                                                         \
-        else                                                                                                     \
-            hipLaunchKernelGGL((k_assemble_all<float, O, P, S>), dim3(nblk(ntot, 256)), dim3(256), 0, ctx->stream, \
-                               a, ntot);                                                                         \
+#define ASM(O, P, S)                                                                                              \
+    case (O - 1) * 4 + (P ? 2 : 0) + (S - 1):                                                                     \
+        if (P && stage) {                                                                                         \
+            if (prec == MFIC_F64)                                                                                 \
+                hipLaunchKernelGGL((k_assemble_all<double, O, P, S, P>), grid, block, stage_lds, ctx->stream, a, ntot); \
+            else                                                                                                  \
+                hipLaunchKernelGGL((k_assemble_all<float, O, P, S, P>), grid, block, stage_lds, ctx->stream, a, ntot); \
+        } else if (prec == MFIC_F64) {                                                                            \
+            hipLaunchKernelGGL((k_assemble_all<double, O, P, S>), grid, block, 0, ctx->stream, a, ntot);           \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((k_assemble_all<float, O, P, S>), grid, block, 0, ctx->stream, a, ntot);            \
+        }                                                                                                         \
         break;
         ASM(1, false, 1) ASM(1, false, 2) ASM(1, true, 1) ASM(1, true, 2)
         ASM(2, false, 1) ASM(2, false, 2) ASM(2, true, 1) ASM(2, true, 2)
