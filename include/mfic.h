@@ -132,7 +132,8 @@ int mfic_fft_get_tile(mfic_ctx ctx, int length, int kind, int prec, int *tile, i
  * per thread and the radix of each Stockham stage); the padded-pass lengths
  * 512 and 768 also have an alternative with twice the elements per thread
  * and 16 threads per transform.  A strided pass of one kind selects its plan
- * per length and precision (the row passes keep the default).  No reference
+ * per length and precision (the row passes keep their own plans); the fused
+ * source row passes select one of two forms the same way.  No reference
  * counterpart: FFTW picks its codelets while planning (grid_fft.cc:53-54). */
 enum mfic_fft_pass {
     MFIC_PASS_PENCILS = 0,    /* strided passes of N-grid transforms (axes 0 and 1) */
