@@ -92,13 +92,11 @@ k_pencils_ax0_shift(cx<T>* __restrict__ data, cx<T>* __restrict__ shifted, int n
             w[e] = cmul(v[e], cx<T>{(T)ph.x, (T)ph.y});
         }
     }
-    fft_pencil<T, L, false, true, V>(v, q, lds, LdsMap{t, tkp}, tw);
+    // both transforms in lockstep: shared twiddle loads and barriers
+    fft_pencil2<T, L, false, V>(v, w, q, reinterpret_cast<T*>(lds), LdsMap{t, tkp}, LdsMap{L * tkp + t, tkp}, tw);
     if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) data[col + (long)(q + P * e) * ps] = cscale(v[e], scale);
-    }
-    fft_pencil<T, L, false, true, V>(w, q, lds, LdsMap{t, tkp}, tw);
-    if (active) {
 #pragma unroll
         for (int e = 0; e < E; ++e) shifted[col + (long)(q + P * e) * ps] = cscale(w[e], scale);
     }
@@ -480,18 +478,19 @@ static void launch_ax0_shift_v(mfic_ctx ctx, cx<T>* data, cx<T>* shifted, int n1
     // tile takes 0.46 ms per launch, the 32-pencil tile 1.37 ms (two outputs
     // per pencil double the registers of the 1024-thread workgroups)
     int tk = 16;
-    while (tk > 4 && ((size_t)L * tk * pencil_lds_elem<T, true>() > pencil_lds_cap<true>() || P * tk > 1024)) tk /= 2;
+    // two exchange regions (both transforms in lockstep)
+    while (tk > 4 && (2 * (size_t)L * tk * pencil_lds_elem<T, true>() > pencil_lds_cap<true>() || P * tk > 1024)) tk /= 2;
     // its own tuning key (kind 2): this pass keeps two outputs per pencil, so
     // a tile tuned for the plain pencils (kind 0) does not carry over
     const auto ov = ctx->tiles.find(std::make_tuple(L, 2, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32));
     if (ov != ctx->tiles.end()) tk = ov->second.first;
-    while (tk > 1 && ((size_t)L * tk * pencil_lds_elem<T, true>() > max_lds(ctx) || P * tk > 1024)) tk /= 2;
+    while (tk > 1 && (2 * (size_t)L * tk * pencil_lds_elem<T, true>() > max_lds(ctx) || P * tk > 1024)) tk /= 2;
     if (V != 0 && tk < 4) tk = 4;  // alternative plans are built for tiles of 4 and more
     const cx<T>* tw = static_cast<const cx<T>*>(twiddles(ctx, L, 0, sizeof(T) == 8 ? MFIC_F64 : MFIC_F32, V));
     const cx<double>* half = static_cast<const cx<double>*>(twiddles(ctx, L, 2, MFIC_F64));
     const int tlen = n1 * nc;
     dim3 grid((unsigned)((tlen + tk - 1) / tk));
-    const size_t lds = (size_t)L * tk * pencil_lds_elem<T, true>();
+    const size_t lds = 2 * (size_t)L * tk * pencil_lds_elem<T, true>();
     ProfScope prof(ctx, "fft_pencils_L" + std::to_string(L) + "_inv_ax0_shift" + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    3.0 * (double)tlen * L * sizeof(cx<T>));
     switch (tk) {

@@ -405,6 +405,81 @@ __device__ __forceinline__ void fft_pencil(cx<T>* v, int q, cx<T>* lds, LdsMap m
     fft_regs<T, L, FWD, LdsMap, BlockSync, SPLIT, V>(v, q, lds, m, tw);
 }
 
+// Two transforms of the same length and positions (registers v and w) in
+// lockstep: each stage applies one set of twiddle loads to both, and the two
+// exchanges (through LDS regions m and m2) share their barriers.  Real and
+// imaginary parts are exchanged in two rounds, as with SPLIT.
+template <typename T, int L, int S, bool FWD, int V>
+__device__ __forceinline__ void fft_stage2(cx<T>* v, cx<T>* w, int q, T* f, LdsMap m, LdsMap m2,
+                                           const cx<T>* __restrict__ tw) {
+    using PL = Plan<L, V>;
+    constexpr int E = PL::E, P = PL::P, R = PL::r(S), B = E / R, NS = plan_ns<L, V>(S);
+    constexpr bool LAST = (S == PL::S - 1);
+    const cx<T>* tws = tw + plan_tw_offset<L, V>(S);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = q + P * b;
+        const int k = j % NS;
+        cx<T> a[R], c[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            a[r] = v[b + r * B];
+            c[r] = w[b + r * B];
+        }
+        if (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const cx<T> t = tws[k * (R - 1) + (r - 1)];
+                a[r] = FWD ? cmul(a[r], t) : cmulc(a[r], t);
+                c[r] = FWD ? cmul(c[r], t) : cmulc(c[r], t);
+            }
+        }
+        Dft<T, R, FWD>::run(a);
+        Dft<T, R, FWD>::run(c);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[b + r * B] = a[r];
+            w[b + r * B] = c[r];
+        }
+    }
+    if constexpr (!LAST) {
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = q + P * b;
+                const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    f[m.at(base + r * NS)] = part ? v[b + r * B].y : v[b + r * B].x;
+                    f[m2.at(base + r * NS)] = part ? w[b + r * B].y : w[b + r * B].x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (part) {
+                    v[e].y = f[m.at(q + P * e)];
+                    w[e].y = f[m2.at(q + P * e)];
+                } else {
+                    v[e].x = f[m.at(q + P * e)];
+                    w[e].x = f[m2.at(q + P * e)];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+template <typename T, int L, bool FWD, int V = 0>
+__device__ __forceinline__ void fft_pencil2(cx<T>* v, cx<T>* w, int q, T* f, LdsMap m, LdsMap m2,
+                                            const cx<T>* __restrict__ tw) {
+    constexpr int S = Plan<L, V>::S;
+    fft_stage2<T, L, 0, FWD, V>(v, w, q, f, m, m2, tw);
+    if constexpr (S > 1) fft_stage2<T, L, 1 % S, FWD, V>(v, w, q, f, m, m2, tw);
+    if constexpr (S > 2) fft_stage2<T, L, 2 % S, FWD, V>(v, w, q, f, m, m2, tw);
+    if constexpr (S > 3) fft_stage2<T, L, 3 % S, FWD, V>(v, w, q, f, m, m2, tw);
+}
+
 // full transform of a row held by one thread group, exchanges through the
 // per-stage swizzled LDS slots starting at `base`
 template <typename T, int L, bool FWD, class SYNC = BlockSync, int V = 0>
