@@ -13,7 +13,10 @@
 // Each transform's threads lie inside one wavefront (groups start on
 // power-of-two lane boundaries), so Stockham stages synchronise per
 // wavefront and idle output groups simply skip the r2c; workgroup barriers
-// separate the three phases.
+// separate the three phases.  Two forms: one half-length transform per row
+// (k_rows_fused) or two rows per full-length complex transform
+// (k_rows_fused_pair, the default), chosen per length and precision at run
+// time.
 #include <cmath>
 
 #include "fft_core.hpp"
