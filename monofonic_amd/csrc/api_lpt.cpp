@@ -397,10 +397,14 @@ int mfic_run_execute(mfic_run r, int blocking) {
         MFIC_CHECK(r, MFIC_ERR_ARG, "null run");
         const mfic_ic_config& c = r->cfg;
         mfic_grid phi1 = r->pot[0];
-        grid_fill(phi1, 0.0);
-        ngenic_fill(phi1, r->d_table);
         phi1->space = MFIC_KSPACE;
-        phi1_from_noise(phi1, phi1, r->eh, r->coef.volfac, c.do_fixing, c.do_inversion);
+        if (phi1_cubic(phi1)) {  // phi1 formed in the noise kernel's stores
+            ngenic_phi1_fill(phi1, r->d_table, phi1_table(phi1, r->eh), r->coef.volfac, c.do_fixing, c.do_inversion);
+        } else {
+            grid_fill(phi1, 0.0);
+            ngenic_fill(phi1, r->d_table);
+            phi1_from_noise(phi1, phi1, r->eh, r->coef.volfac, c.do_fixing, c.do_inversion);
+        }
         if (c.lpt_order > 1) lpt_sources(r->ctx, c.lpt_order, phi1, r->pot[1], r->pot[2], r->pot[3], r->pot[4], r->pot[5]);
         const void* phi[3] = {phi1->d, r->pot[1] ? r->pot[1]->d : nullptr, r->pot[2] ? r->pot[2]->d : nullptr};
         const void* A3[3] = {r->pot[3] ? r->pot[3]->d : nullptr, r->pot[4] ? r->pot[4]->d : nullptr,

@@ -199,6 +199,24 @@ void grid_phase_shift(mfic_grid g, const double s[3]) {
     MFIC_LAUNCH_CHECK();
 }
 
+bool phi1_cubic(mfic_grid phi) {
+    const KGeom G = geom_of(phi);
+    return G.n0 == G.n1 && G.n1 == G.n2 && G.kf0 == G.kf1 && G.kf1 == G.kf2;
+}
+
+const double* phi1_table(mfic_grid phi, const EhParams& eh, double* table) {
+    MFIC_CHECK(eh.tk_n == 0 || eh.tk != nullptr, MFIC_ERR_STATE, "tabulated transfer function not on the device");
+    MFIC_CHECK(phi1_cubic(phi), MFIC_ERR_SIZE, "the -A(k)/k^2 table needs a cubic box");
+    const KGeom G = geom_of(phi);
+    const int nmax = 3 * (G.n0 / 2) * (G.n0 / 2), bs = 256;
+    double* tab = table ? table
+                        : static_cast<double*>(ctx_scratch(phi->ctx, "phi1_tab", (size_t)(nmax + 1) * sizeof(double)));
+    hipLaunchKernelGGL(k_phi1_table, dim3(nblocks((size_t)nmax + 1, bs)), dim3(bs), 0, phi->ctx->stream, tab, nmax,
+                       G.kf0, eh);
+    MFIC_LAUNCH_CHECK();
+    return tab;
+}
+
 void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double volfac, int fixing, int inversion,
                      double* table) {
     MFIC_CHECK(eh.tk_n == 0 || eh.tk != nullptr, MFIC_ERR_STATE, "tabulated transfer function not on the device");
@@ -206,13 +224,8 @@ void phi1_from_noise(mfic_grid phi, mfic_grid wn, const EhParams& eh, double vol
     const size_t nc = (size_t)G.n0 * G.n1 * G.nc;
     const int bs = 256;
     ProfScope prof(phi->ctx, "phi1", 2.0 * phi->bytes());
-    if (G.n0 == G.n1 && G.n1 == G.n2 && G.kf0 == G.kf1 && G.kf1 == G.kf2) {
-        const int nmax = 3 * (G.n0 / 2) * (G.n0 / 2);
-        double* tab = table ? table
-                            : static_cast<double*>(ctx_scratch(phi->ctx, "phi1_tab", (size_t)(nmax + 1) * sizeof(double)));
-        hipLaunchKernelGGL(k_phi1_table, dim3(nblocks((size_t)nmax + 1, bs)), dim3(bs), 0, phi->ctx->stream, tab, nmax,
-                           G.kf0, eh);
-        MFIC_LAUNCH_CHECK();
+    if (phi1_cubic(phi)) {
+        const double* tab = phi1_table(phi, eh, table);
         if (phi->prec == MFIC_F64)
             hipLaunchKernelGGL(k_phi1_tab<double>, dim3(nblocks(nc, bs)), dim3(bs), 0, phi->ctx->stream,
                                (cx<double>*)phi->d, (const cx<double>*)wn->d, tab, G, nc, volfac, fixing, inversion);

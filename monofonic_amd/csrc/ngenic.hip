@@ -266,9 +266,29 @@ __device__ __forceinline__ cx<T> ngenic_value(double phase, double ampl, const C
 // leave the workgroup as runs of NGENIC_KT consecutive elements per column,
 // so every cache line is written whole.
 constexpr int NGENIC_BS = 64, NGENIC_KT = 16;
+// PHI: store phi1 = sg * w / volfac * F(|n|^2) instead of the noise w (the
+// values k_phi1_tab forms from the stored noise, from the same T-rounded w),
+// with F the FP64 -A(k)/k^2 table per integer |n|^2 of a cubic box and w
+// normalised to |w| = 1 when fixing
+struct Phi1Fold {
+    const double* tab = nullptr;
+    double sg = 1.0, volfac = 1.0;
+    int fixing = 0;
+};
 template <typename T>
+__device__ __forceinline__ cx<T> phi1_fold(cx<T> w, int si, int sj, int k, const Phi1Fold& f) {
+    double wr = w.x, wi = w.y;
+    if (f.fixing) {
+        const double a = hypot(wr, wi);
+        if (a != 0.0) { wr /= a; wi /= a; }
+    }
+    const double F = f.tab[si * si + sj * sj + k * k];
+    return cx<T>{(T)(f.sg * wr / f.volfac * F), (T)(f.sg * wi / f.volfac * F)};
+}
+
+template <typename T, bool PHI = false>
 __global__ void __launch_bounds__(NGENIC_BS)
-k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, CrTables tabs) {
+k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, CrTables tabs, Phi1Fold pf) {
     __shared__ cx<T> tile[NGENIC_BS][NGENIC_KT + 1];
     const long cols = (long)n * n;
     const long col0 = (long)blockIdx.x * NGENIC_BS;
@@ -293,7 +313,9 @@ k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, 
         double phase, ampl;
         draw(phase, ampl);
         if (!nyq_col && !(i == 0 && j == 0) && (i == 0 ? j < nh : i < nh)) {
-            const cx<T> z = ngenic_value<T>(phase, ampl, tabs);
+            cx<T> z = ngenic_value<T>(phase, ampl, tabs);
+            // the mirror (-i, -j) has the same |n|^2: its value is conj(z) either way
+            if constexpr (PHI) z = phi1_fold(z, i - (i > nh) * n, j - (j > nh) * n, 0, pf);
             g[((long)i * n + j) * nc] = z;
             g[((long)ii * n + jj) * nc] = cx<T>{z.x, -z.y};
         }
@@ -305,7 +327,11 @@ k_ngenic_fill(cx<T>* __restrict__ g, const uint32_t* __restrict__ table, int n, 
                 if (k >= nc) break;
                 double phase, ampl;
                 draw(phase, ampl);
-                tile[threadIdx.x][kk] = (nyq_col || k == nh) ? cx<T>{0, 0} : ngenic_value<T>(phase, ampl, tabs);
+                cx<T> z = (nyq_col || k == nh) ? cx<T>{0, 0} : ngenic_value<T>(phase, ampl, tabs);
+                if constexpr (PHI) {
+                    if (!(nyq_col || k == nh)) z = phi1_fold(z, i - (i > nh) * n, j - (j > nh) * n, k, pf);
+                }
+                tile[threadIdx.x][kk] = z;
             }
         }
         __syncthreads();
@@ -359,10 +385,44 @@ void ngenic_fill(mfic_grid g, const uint32_t* dev_table) {
     ProfScope prof(g->ctx, "ngenic_fill", (double)cols * (n / 2 + 1) * (g->prec == MFIC_F64 ? 16 : 8));
     if (g->prec == MFIC_F64)
         hipLaunchKernelGGL(k_ngenic_fill<double>, dim3((unsigned)((cols + bs - 1) / bs)), dim3(bs), 0, g->ctx->stream,
-                           (cx<double>*)g->d, dev_table, n, tabs);
+                           (cx<double>*)g->d, dev_table, n, tabs, Phi1Fold{});
     else
         hipLaunchKernelGGL(k_ngenic_fill<float>, dim3((unsigned)((cols + bs - 1) / bs)), dim3(bs), 0, g->ctx->stream,
-                           (cx<float>*)g->d, dev_table, n, tabs);
+                           (cx<float>*)g->d, dev_table, n, tabs, Phi1Fold{});
+    MFIC_LAUNCH_CHECK();
+}
+
+// zero the kz = 0 plane of a complex [n][n][n/2+1] grid (the only entries
+// the noise kernel does not write to every column of)
+template <typename T> __global__ void k_zero_kz0(cx<T>* __restrict__ g, long cols, int nc) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cols) g[c * nc] = cx<T>{0, 0};
+}
+
+void ngenic_phi1_fill(mfic_grid phi, const uint32_t* dev_table, const double* phitab, double volfac, int fixing,
+                      int inversion) {
+    MFIC_CHECK(phi->n[0] == phi->n[1] && phi->n[1] == phi->n[2], MFIC_ERR_SIZE, "NGenIC needs a cubic grid");
+    MFIC_CHECK(phi->n[0] % 2 == 0, MFIC_ERR_SIZE, "NGenIC needs an even grid size");
+    static const CrTables tabs = make_tables();
+    const int n = (int)phi->n[0], nc = n / 2 + 1;
+    const long cols = (long)n * n;
+    const int bs = NGENIC_BS;
+    Phi1Fold pf;
+    pf.tab = phitab;
+    pf.sg = inversion ? -1.0 : 1.0;
+    pf.volfac = volfac;
+    pf.fixing = fixing;
+    ProfScope prof(phi->ctx, "ngenic_phi1", (double)cols * nc * (phi->prec == MFIC_F64 ? 16 : 8));
+    const dim3 zgrid((unsigned)((cols + 255) / 256)), grid((unsigned)((cols + bs - 1) / bs));
+    if (phi->prec == MFIC_F64) {
+        hipLaunchKernelGGL(k_zero_kz0<double>, zgrid, dim3(256), 0, phi->ctx->stream, (cx<double>*)phi->d, cols, nc);
+        hipLaunchKernelGGL((k_ngenic_fill<double, true>), grid, dim3(bs), 0, phi->ctx->stream, (cx<double>*)phi->d,
+                           dev_table, n, tabs, pf);
+    } else {
+        hipLaunchKernelGGL(k_zero_kz0<float>, zgrid, dim3(256), 0, phi->ctx->stream, (cx<float>*)phi->d, cols, nc);
+        hipLaunchKernelGGL((k_ngenic_fill<float, true>), grid, dim3(bs), 0, phi->ctx->stream, (cx<float>*)phi->d,
+                           dev_table, n, tabs, pf);
+    }
     MFIC_LAUNCH_CHECK();
 }
 
