@@ -62,6 +62,32 @@ def test_fft_alternative_radix_plan(ctx, shape, prec, tol):
 
 
 @pytest.mark.gpu
+def test_plan_and_tile_keys(ctx):
+    """the fused-row form and tile overrides read back as set, restore to the
+    built-in choice, and out-of-range choices are refused"""
+    import ctypes as C
+    from monofonic_amd import _lib as L
+    from monofonic_amd import batch
+    batch.set_plan(ctx, 768, L.PASS_FUSED_ROWS, F32, 1)
+    assert batch.get_plan(ctx, 768, L.PASS_FUSED_ROWS, F32) == 1
+    assert batch.get_plan(ctx, 768, L.PASS_FUSED_ROWS, F64) == 0
+    with pytest.raises(L.MficError):
+        batch.set_plan(ctx, 768, L.PASS_FUSED_ROWS, F32, 2)
+    batch.set_plan(ctx, 768, L.PASS_FUSED_ROWS, F32, -1)
+    assert batch.get_plan(ctx, 768, L.PASS_FUSED_ROWS, F32) == 0
+    t, pad = C.c_int(), C.c_int()
+    L.call("mfic_fft_set_tile", ctx.h, 768, L.PASS_TRUNC_AX1, F32, 4, 0)
+    L.call("mfic_fft_get_tile", ctx.h, 768, L.PASS_TRUNC_AX1, F32, C.byref(t), C.byref(pad))
+    assert (t.value, pad.value) == (4, 0)
+    assert batch.chosen_tiles(ctx, 512, F32)[(768, L.PASS_TRUNC_AX1)] == 4
+    L.call("mfic_fft_set_tile", ctx.h, 768, L.PASS_TRUNC_AX1, F32, 0, 0)
+    L.call("mfic_fft_get_tile", ctx.h, 768, L.PASS_TRUNC_AX1, F32, C.byref(t), C.byref(pad))
+    assert t.value == 0 and (768, L.PASS_TRUNC_AX1) not in batch.chosen_tiles(ctx, 512, F32)
+    with pytest.raises(L.MficError):
+        L.call("mfic_fft_set_tile", ctx.h, 768, L.PASS_TRUNC_AX1, F32, 3, 0)
+
+
+@pytest.mark.gpu
 def test_fft_single_mode(ctx):
     """a single cosine mode maps to two half-amplitude delta peaks"""
     n = 64
