@@ -114,11 +114,11 @@ def test_potentials_match_oracle(ctx, order, prec):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [24, 64, 96, 192, 256])
+@pytest.mark.parametrize("n", [8, 12, 16, 24, 64, 96, 128, 192, 256])
 def test_potentials_3lpt_sizes(ctx, n):
-    """3LPT potentials across padded row lengths: one / several transforms
-    per wavefront, lane slots wider than a transform (96^3, 192^3) and the
-    16-lane transforms of 256^3"""
+    """3LPT potentials across padded row lengths (12 to 384 points): one /
+    several transforms per wavefront, lane slots wider than a transform
+    (96^3, 192^3), single-thread transforms of the smallest rows"""
     box, z, seed = 2.0 * n, 24.0, 99
     cfg = O.ic_cfg(n, box, 3, seed)
     oc = O.baseline_cosmology(z)
