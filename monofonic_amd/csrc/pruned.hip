@@ -265,13 +265,14 @@ template <typename T> struct StUnpad {
         if (i < 0) return;
         double re = 0.0, im = 0.0;
         if (!(c.nyq || i == G::NH)) {
-            re = (double)v.x * scale;
-            im = (double)v.y * scale;
-            if (invlap) {
+            double f = scale;
+            if (invlap) {  // one FP64 division per element, shared by both parts
                 const double ki = G::template grad<double>(q, e, kf);
                 const double k2 = ki * ki + c.kj * c.kj + c.kk * c.kk;
-                if (k2 > 0.0) { re = -re / k2; im = -im / k2; } else { re = 0.0; im = 0.0; }
+                f = k2 > 0.0 ? -scale / k2 : 0.0;
             }
+            re = (double)v.x * f;
+            im = (double)v.y * f;
         }
         cx<T>* dst = c.p + (long)i * (G::N * G::KC);
         if (op == MFIC_OP_ASSIGN) { *dst = cx<T>{(T)re, (T)im}; return; }
