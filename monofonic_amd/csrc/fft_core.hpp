@@ -284,20 +284,30 @@ MFIC_SWZ(512, 1, 1, 5, 0, 6, 8)
 MFIC_SWZ_V(768, 0, 3, 1, 3, 2, 5, 1)
 MFIC_SWZ_V(768, 1, 3, 1, 5, 0, 5, 4)
 MFIC_SWZ_V(768, 0, 2, 1, 4, 1, 6, 1)
-MFIC_SWZ_V(768, 1, 2, 1, 4, 0, 4, 5)
+MFIC_SWZ_V(768, 1, 2, 1, 4, 1, 4, 4)
 MFIC_SWZ_V(768, 2, 2, 1, 6, 16, 6, 0)
 #undef MFIC_SWZ
 #undef MFIC_SWZ_V
+__host__ __device__ constexpr bool pow2_or_0(int k) { return (k & (k - 1)) == 0; }
 template <int L, int S, int V = 0> struct LdsSwzMap {
+    using W = RowSwz<L, S, V>;
+    static_assert(W::kind == 0 || L % (1 << W::a) == 0, "swizzle must stay inside the row");
     int base;
-    __device__ __forceinline__ int at(int x) const {
-        using W = RowSwz<L, S, V>;
-        static_assert(W::kind == 0 || L % (1 << W::a) == 0, "swizzle must stay inside the row");
+    __host__ __device__ static constexpr int swz(int x) {
         constexpr int A = (1 << W::a) - 1;
-        if constexpr (W::kind == 0) return base + x;
-        else if constexpr (W::kind == 1) return base + (x ^ ((((x >> W::a) * W::k1) ^ ((x >> W::b) * W::k2)) & A));
-        else return base + ((x & ~A) | ((x + (x >> W::a) * W::k1 + (x >> W::b) * W::k2) & A));
+        if constexpr (W::kind == 0) return x;
+        else if constexpr (W::kind == 1) return x ^ ((((x >> W::a) * W::k1) ^ ((x >> W::b) * W::k2)) & A);
+        else return (x & ~A) | ((x + (x >> W::a) * W::k1 + (x >> W::b) * W::k2) & A);
     }
+    // an xor swizzle whose multipliers are 0 or powers of two is linear over
+    // GF(2): swz(u + c) = swz(u) ^ swz(c) when u and c share no bits, so an
+    // address is one per-thread value xor a compile-time constant
+    static constexpr bool LINEAR = W::kind == 0 || (W::kind == 1 && pow2_or_0(W::k1) && pow2_or_0(W::k2));
+    __device__ __forceinline__ int at(int x) const { return base + swz(x); }
+};
+template <class MAP> struct map_linear { static constexpr bool value = false; };
+template <int L, int S, int V> struct map_linear<LdsSwzMap<L, S, V>> {
+    static constexpr bool value = LdsSwzMap<L, S, V>::LINEAR;
 };
 
 // LDS exchange synchronisation between Stockham stages
@@ -324,6 +334,8 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, co
     using PL = Plan<L, V>;
     constexpr int E = PL::E, P = PL::P, R = PL::r(S), B = E / R, NS = plan_ns<L, V>(S);
     constexpr bool LAST = (S == PL::S - 1);
+    constexpr bool LINW = map_linear<MAP>::value && pow2_or_0(NS) && pow2_or_0(R);
+    constexpr bool LINR = map_linear<MAP>::value && pow2_or_0(P);
     const cx<T>* tws = tw + plan_tw_offset<L, V>(S);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -343,6 +355,10 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, co
         if (LAST || SPLIT) {  // outputs stay on this thread's registers
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b + r * B] = a[r];
+        } else if constexpr (LINW) {  // r * NS and the thread's part share no bits
+            const int fu = MAP::swz((j / NS) * NS * R + k);
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[m.base + (fu ^ MAP::swz(r * NS))] = a[r];
         } else {
             const int base = (j / NS) * NS * R + k;
 #pragma unroll
@@ -372,6 +388,12 @@ __device__ __forceinline__ void fft_stage(cx<T>* v, int q, cx<T>* lds, MAP m, co
         SYNC::sync();
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e].y = f[m.at(q + P * e)];
+        SYNC::sync();
+    } else if constexpr (!LAST && LINR) {  // q < P and P * e share no bits
+        SYNC::sync();
+        const int fq = MAP::swz(q);
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = lds[m.base + (fq ^ MAP::swz(P * e))];
         SYNC::sync();
     } else if constexpr (!LAST) {
         SYNC::sync();
