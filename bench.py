@@ -551,11 +551,21 @@ def main():
         log("FP64 form of the same workload")
         cfg64 = ICConfig(n=n, box=box, zstart=24.0, lpt_order=3, lattice=BCC, plt=True, prec=F64, seed=12345,
                          particles=True)
+        tune64 = None
+        if not args.no_tune:  # the FP64 plans and tiles are keys of their own
+            from monofonic_amd.batch import autotune_config, chosen_plans, chosen_tiles
+            from monofonic_amd.lpt import baseline_cosmology
+            log("autotune radix plans and tiles for the FP64 run (plan time)")
+            t64 = autotune_config(ctx, cfg64, baseline_cosmology(24.0))
+            tune64 = {"tuned_ms": round(t64, 3),
+                      "plans": {f"L{k[0]}_pass{k[1]}": v for k, v in chosen_plans(ctx, n, F64).items()},
+                      "tiles": {f"L{k[0]}_kind{k[1]}": v for k, v in chosen_tiles(ctx, n, F64).items()}}
         run64 = ICRun(ctx, cfg64)
         ms64, rep64 = time_steps(ctx, run64, max(2, args.steps // 2), 1, profile=True)
         run64.close()
         line["fp64"] = {"ms_per_step": round(ms64, 3), "value": round(n ** 3 / (ms64 * 1e-3), 1),
-                        "fp32_speedup": round(ms64 / ms, 3), "roofline": roofline(rep64, max(2, args.steps // 2))}
+                        "fp32_speedup": round(ms64 / ms, 3), "autotune": tune64,
+                        "roofline": roofline(rep64, max(2, args.steps // 2))}
         log(f"FP64: {ms64:.2f} ms/step")
     if not args.no_accuracy:
         log("accuracy: FP32 vs FP64 device particles (Psi = pos - q, v)")
