@@ -241,21 +241,27 @@ k_rows_fused(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out
 // forward transform as O_2p[k] = W[k] + conj(W[L2-k]),
 // O_2p+1[k] = -i (W[k] - conj(W[L2-k])) (the 1/2 and both normalisations
 // are one FP64 factor on the products).
-template <int L2, int GI> constexpr int pair_rows_per_block() {
-    constexpr int tpr = GI * lane_slot<Plan<L2, pair_in_plan<L2>()>::P>();
+// input transforms on the 32-lane plan; FP64 on the 64-lane output plan
+// (twice the wavefronts per row: the FP64 passes hold one or two workgroups
+// per CU, so their latency wants the waves)
+template <typename T, int L2> constexpr int pair_in_variant() {
+    return sizeof(T) == 8 && pair_out_plan<L2>() != 0 ? pair_out_plan<L2>() : pair_in_plan<L2>();
+}
+template <typename T, int L2, int GI> constexpr int pair_rows_per_block() {
+    constexpr int tpr = GI * lane_slot<Plan<L2, pair_in_variant<T, L2>()>::P>();
     return tpr >= 256 ? 1 : (256 / tpr > 8 ? 8 : 256 / tpr);
 }
-template <int L2, class PR> constexpr int pair_block_threads() {
+template <typename T, int L2, class PR> constexpr int pair_block_threads() {
     constexpr int GI = PR::NI / 2;
-    return pair_rows_per_block<L2, GI>() * GI * lane_slot<Plan<L2, pair_in_plan<L2>()>::P>();
+    return pair_rows_per_block<T, L2, GI>() * GI * lane_slot<Plan<L2, pair_in_variant<T, L2>()>::P>();
 }
 // output transforms on 64 lanes (plan pair_out_plan) where every output
 // pair of the block gets a wavefront, else on the input transforms' lanes
-template <int L2, class PR> constexpr int pair_out_variant() {
-    constexpr int GO = (PR::NO + 1) / 2, RB = pair_rows_per_block<L2, PR::NI / 2>();
+template <typename T, int L2, class PR> constexpr int pair_out_variant() {
+    constexpr int GO = (PR::NO + 1) / 2, RB = pair_rows_per_block<T, L2, PR::NI / 2>();
     constexpr int VO = pair_out_plan<L2>();
-    return VO != 0 && RB * GO * lane_slot<Plan<L2, VO>::P>() <= pair_block_threads<L2, PR>() ? VO
-                                                                                            : pair_in_plan<L2>();
+    return VO != 0 && RB * GO * lane_slot<Plan<L2, VO>::P>() <= pair_block_threads<T, L2, PR>() ? VO
+                                                                                               : pair_in_variant<T, L2>();
 }
 
 // LDS slots per pair row: the first pitch of at least L2 that is 32 modulo
@@ -264,15 +270,15 @@ template <int L2, class PR> constexpr int pair_out_variant() {
 template <int L2> constexpr int pair_row_pitch() { return L2 + ((32 - L2 % 64) % 64 + 64) % 64; }
 
 template <typename T, int L2, class PR>
-__global__ void __launch_bounds__((pair_block_threads<L2, PR>()))
+__global__ void __launch_bounds__((pair_block_threads<T, L2, PR>()))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? PR::WAVES : 1, 8)))
 k_rows_fused_pair(const cx<T>* __restrict__ in, long istride, cx<T>* __restrict__ out, long ostride, long nrows,
                   double scale2, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ two) {
     constexpr int NI = PR::NI, NO = PR::NO, GI = NI / 2, GO = (NO + 1) / 2;
-    constexpr int VI = pair_in_plan<L2>(), VO = pair_out_variant<L2, PR>();
+    constexpr int VI = pair_in_variant<T, L2>(), VO = pair_out_variant<T, L2, PR>();
     constexpr int E = Plan<L2, VI>::E, P = Plan<L2, VI>::P, SL = lane_slot<P>();
     constexpr int EO = Plan<L2, VO>::E, PO = Plan<L2, VO>::P, SLO = lane_slot<PO>();
-    constexpr int RB = pair_rows_per_block<L2, GI>(), TPR = GI * SL, RP = pair_row_pitch<L2>();
+    constexpr int RB = pair_rows_per_block<T, L2, GI>(), TPR = GI * SL, RP = pair_row_pitch<L2>();
     constexpr int KC = L2 / 3 + 1;  // N/2 + 1 values per compact row, N = 2 L2 / 3
     constexpr int NPOS = (L2 + TPR - 1) / TPR;
     static_assert(NI % 2 == 0, "inputs come in pairs");
@@ -393,7 +399,7 @@ template <typename T, int L2, class PR>
 static void launch_fused_pair(mfic_ctx ctx, const void* in, void* out, long stride, int kc, long nrows, double scale,
                               const char* name) {
     constexpr int NI = PR::NI, NO = PR::NO, GI = NI / 2;
-    constexpr int RB = pair_rows_per_block<L2, GI>(), VI = pair_in_plan<L2>(), VO = pair_out_variant<L2, PR>();
+    constexpr int RB = pair_rows_per_block<T, L2, GI>(), VI = pair_in_variant<T, L2>(), VO = pair_out_variant<T, L2, PR>();
     MFIC_CHECK(3 * (kc - 1) == L2, MFIC_ERR_SIZE, "half-spectrum rows must hold N/2 + 1 values of a 3N/2 padded row");
     const size_t lds = (size_t)RB * GI * pair_row_pitch<L2>() * sizeof(cx<T>);
     MFIC_CHECK(lds <= (size_t)ctx->max_lds, MFIC_ERR_SIZE, "fused row pass exceeds the LDS of a workgroup");
@@ -403,7 +409,7 @@ static void launch_fused_pair(mfic_ctx ctx, const void* in, void* out, long stri
     dim3 grid((unsigned)xcd_grid((nrows + RB - 1) / RB));
     ProfScope prof(ctx, std::string(name) + "_n" + std::to_string(L2) + (sizeof(T) == 8 ? "_f64" : "_f32"),
                    (double)nrows * (NI + NO) * kc * sizeof(cx<T>));
-    hipLaunchKernelGGL((k_rows_fused_pair<T, L2, PR>), grid, dim3(pair_block_threads<L2, PR>()), lds, ctx->stream,
+    hipLaunchKernelGGL((k_rows_fused_pair<T, L2, PR>), grid, dim3(pair_block_threads<T, L2, PR>()), lds, ctx->stream,
                        static_cast<const cx<T>*>(in), stride, static_cast<cx<T>*>(out), stride, nrows,
                        0.5 * scale * scale, tw, two);
     MFIC_LAUNCH_CHECK();
